@@ -1,0 +1,193 @@
+"""Benchmark: B-spline evals/s at depth 8 (plus LSCG time to 1e-6).
+
+Workload: the (3,5) torus-knot tube with 200,000 triangles and seeded vertex
+jitter N(0, 1e-3), normalised to the unit cube.  The octree has depth 8
+(256^3 voxels, 9 levels).  The coefficients come from the GPU LSCG fit of
+the signed distance at the node vertices.  There are 10^7 binary32 query
+points, uniform in [-0.05, 1.05]^3, already resident in HBM.
+
+A step is one evaluation of f at all 10^7 points (its_bspline_eval_device,
+support semantics of the reference CPU query path).  The JSON line also
+carries:
+- the cold-start LSCG time to a 1e-6 normal residual on the same octree;
+- the roofline of the evaluation kernel (algorithmic bytes / measured launch
+  time);
+- the oracle CPU path, timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_QUERIES = 10_000_000
+R = 256
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_mesh():
+    from its_amd import _capi, meshes
+
+    V, F = meshes.torus_knot_tube()  # 2000 x 50 x 2 = 200,000 triangles
+    V = _capi.model_to_unit_cube(V)
+    V = meshes.jitter(V, 1e-3, seed=7)
+    origin, end, width = _capi.uniform_bbox(V)
+    return V, F, origin, end, width
+
+
+def cpu_baseline(V, F, origin, end, width, lam, q_host, budget_s=15.0):
+    """Oracle (FP64 CPU restatement of mt_cpuTest_2) on a prefix of the queries."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from its_amd import meshes
+
+    svo = O.SVO(meshes.triangles(V, F), origin, width, R)
+    threads = O.lib().ora_num_threads()
+    n = 20000
+    t0 = time.perf_counter()
+    svo.eval_support(lam, q_host[:n].astype(np.float64), end, 0.0)
+    dt = time.perf_counter() - t0
+    n2 = int(min(len(q_host), max(n, budget_s / max(dt, 1e-9) * n)))
+    t0 = time.perf_counter()
+    svo.eval_support(lam, q_host[:n2].astype(np.float64), end, 0.0)
+    dt2 = time.perf_counter() - t0
+    return {
+        "value": n2 / dt2,
+        "unit": "evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {n2} of the {N_QUERIES} queries, oracle support evaluation (FP64, OpenMP)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from its_amd import _capi
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    stream = torch.cuda.current_stream(dev)
+    ctx = _capi.Context(0)
+
+    log("[bench] mesh")
+    V, F, origin, end, width = make_mesh()
+    from its_amd import meshes
+
+    tris = meshes.triangles(V, F)
+    t0 = time.perf_counter()
+    svo = _capi.Octree(ctx, tris, origin, width, R)
+    t_svo = time.perf_counter() - t0
+    log(f"[bench] octree depth={svo.depth} nodes={svo.num_nodes} verts={svo.num_verts} ({t_svo:.2f}s)")
+    t0 = time.perf_counter()
+    b = svo.sdf_at_vertices(V, F)
+    t_sdf = time.perf_counter() - t0
+    log(f"[bench] rhs ({t_sdf:.2f}s)")
+    # warm-up fit builds the operator structure and the solver graph
+    svo.lscg(b, tol=1e-6)
+    lscg_runs = []
+    for _ in range(3):
+        lam, st = svo.lscg(b, tol=1e-6)
+        lscg_runs.append(st)
+    t_lscg = float(np.median([s["seconds"] for s in lscg_runs]))
+    st = lscg_runs[-1]
+    log(f"[bench] lscg cold: {t_lscg*1e3:.2f} ms, iters={st['iterations']} normal_rel={st['normal_rel']:.2e}")
+
+    # queries resident in HBM
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234)
+    q = torch.rand((3, N_QUERIES), generator=g, device=dev, dtype=torch.float32) * 1.1 - 0.05
+    out = torch.empty(N_QUERIES, device=dev, dtype=torch.float32)
+    ptr = [q[0].data_ptr(), q[1].data_ptr(), q[2].data_ptr()]
+
+    def step():
+        svo.eval_device(ptr[0], ptr[1], ptr[2], N_QUERIES, end, out.data_ptr(), _capi.EVAL_SUPPORT, 0.0,
+                        stream=stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    launch_ms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    avg_ms = float(np.mean(launch_ms))
+    med_ms = float(np.median(launch_ms))
+    value = N_QUERIES * args.steps / elapsed
+
+    # roofline of the evaluation kernel: 16 B/query (12 B of binary32 xyz in,
+    # 4 B of f out) + 8 B per unknown (one pass over the coefficient field)
+    alg_bytes = 16.0 * N_QUERIES + 8.0 * svo.num_verts
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "eval_knot256_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline(V, F, origin, end, width, lam, q.cpu().numpy().T.copy())
+
+    res = {
+        "metric": "B-spline evals/s @depth 8; LSCG time-to-1e-6 residual, warm vs cold start",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "torus-knot (3,5) tube, 200k triangles, jitter N(0,1e-3), SVO depth 8 (256^3), "
+                        "10^7 FP32 queries uniform in [-0.05,1.05]^3",
+            "queries": N_QUERIES,
+            "svo_R": R,
+            "tree_depth": svo.depth,
+            "unknowns": svo.num_verts,
+            "nodes": svo.num_nodes,
+        },
+        "eval_kernel_ms": {"mean": avg_ms, "median": med_ms},
+        "lscg_cold": {"seconds": t_lscg, "iterations": st["iterations"], "normal_rel": st["normal_rel"],
+                      "residual_rel": st["residual_rel"]},
+        "lscg_warm": None,
+        "setup_seconds": {"svo_build": t_svo, "sdf_rhs": t_sdf},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

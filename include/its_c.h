@@ -1,0 +1,179 @@
+/*
+ * its_c.h -- C ABI of the MI355X (gfx950) implementation of the ITS hot path.
+ *
+ * Conventions
+ *   - Every function returns ITS_OK (0) or a negative ITS_E_* code; the text of
+ *     the last error on the calling thread is returned by its_last_error().
+ *     Nothing crosses this boundary as an exception or exit().
+ *   - Host pointers (suffix-free arguments) are caller owned; the library copies
+ *     what it needs.  Entry points with the suffix _device take device pointers
+ *     and a hipStream_t (passed as void*; NULL means the context stream) and do
+ *     not synchronise.
+ *   - Opaque handles own device memory; one context per host thread.
+ *   - Floating point: octree geometry is binary64 and bit-identical to the CPU
+ *     restatement of the reference; B-spline evaluation and the LSCG matvec run
+ *     in binary32 with binary64 reductions, residuals and refinement.
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef ITS_C_H
+#define ITS_C_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ITS_OK 0
+#define ITS_E_INVALID (-1)   /* bad argument */
+#define ITS_E_HIP (-2)       /* HIP runtime error */
+#define ITS_E_NOMEM (-3)     /* device or host allocation failed */
+#define ITS_E_STATE (-4)     /* call out of order (e.g. eval before a fit) */
+#define ITS_E_RANGE (-5)     /* size beyond a documented limit */
+#define ITS_E_NOTFOUND (-6)  /* a point's fine voxel is not an octree node */
+
+#define ITS_NONE 0xFFFFFFFFu /* "no node" (UINT_MAX in the reference) */
+#define ITS_MAX_LEVELS 12
+
+typedef struct its_ctx_s* its_ctx;
+typedef struct its_svo_s* its_svo;
+
+const char* its_last_error(void);
+const char* its_version(void);
+
+/* ---- context (replaces the per-call device selection and cudaMalloc/free
+ *      of src/cuAcc/CUDACompute.cu:950-982 and DeviceQuery.cuh:41-56) ---- */
+int its_ctx_create(int device, its_ctx* out);
+int its_ctx_destroy(its_ctx ctx);
+/* make the library launch on an external stream (e.g. torch's current stream) */
+int its_ctx_set_stream(its_ctx ctx, void* hip_stream);
+int its_ctx_synchronize(its_ctx ctx);
+int its_device_count(int* count);
+
+/* ---- model normalisation and grid box (BaseModel::model2UnitCube,
+ *      BaseModel.cpp:84-92; BaseModel::setUniformBoundingBox, 24-51) ---- */
+int its_model_to_unit_cube(double* V, int64_t nV, double scale_factor);
+int its_uniform_bbox(const double* V, int64_t nV, double origin[3], double end[3], double width[3]);
+
+/* ---- sparse voxel octree (SparseVoxelOctree::createOctree, SVO.h:104,
+ *      SVO.cu:674-856, 858-1113) ----
+ * tris: T x 9 binary64 (p1, p2, p3), as the Triangle<V3d> list.
+ * origin/width: the grid box (AABox boxOrigin / boxWidth).  R: voxels per axis,
+ * a power of two in [2, 1024] (the reference's surfaceVoxelGridSize, cubic). */
+int its_svo_build(its_ctx ctx, const double* tris, int64_t T, const double origin[3], const double width[3], int R,
+                  its_svo* out);
+int its_svo_build_device(its_ctx ctx, const double* d_tris, int64_t T, const double origin[3], const double width[3],
+                         int R, its_svo* out);
+int its_svo_destroy(its_svo svo);
+
+typedef struct {
+    int tree_depth;                               /* treeDepth (levels), finest = 0 */
+    int R;
+    int64_t num_voxels;                           /* numVoxels */
+    int64_t num_nodes;                            /* numTreeNodes */
+    int64_t num_verts;                            /* numNodeVerts (= unknowns n) */
+    int64_t depth_num_nodes[ITS_MAX_LEVELS];      /* depthNumNodes */
+    int64_t esum_nodes[ITS_MAX_LEVELS + 1];
+    int64_t esum_verts[ITS_MAX_LEVELS + 1];       /* esumDepthNodeVerts */
+    double grid_origin[3];
+    double voxel_width;                           /* svoNodeArray[0].width */
+} its_svo_info;
+int its_svo_get_info(its_svo svo, its_svo_info* info);
+
+/* SVONode fields (SVO.h:15-32) as SoA, global order finest -> root.  Any
+ * pointer may be NULL.  childs: 8 per node, neighbors: 27 per node. */
+int its_svo_export_nodes(its_svo svo, uint32_t* morton, uint8_t* is_leaf, double* origin, double* width,
+                         uint32_t* parent, uint32_t* childs, uint32_t* neighbors);
+/* nodeVertexArray (SVO.h:53): corner position and owner node, level by level */
+int its_svo_export_verts(its_svo svo, double* xyz, uint32_t* owner);
+/* sorted occupied-voxel Morton codes (the compacted level-0 array) */
+int its_svo_export_voxels(its_svo svo, uint32_t* morton);
+
+/* ---- right-hand side: signed distance at every node vertex
+ *      (ThinShells::cpSDFOfTreeNodes, ThinShells.cpp:481-509) ----
+ * V: nV x 3, F: nF x 3.  b: n = num_verts values (host).  Also keeps the
+ * result and the nearest-triangle index on the device for the fit. */
+int its_sdf_at_vertices(its_ctx ctx, its_svo svo, const double* V, int64_t nV, const int32_t* F, int64_t nF,
+                        double* b);
+/* general points (N x 3) */
+int its_signed_distance(its_ctx ctx, const double* V, int64_t nV, const int32_t* F, int64_t nF, const double* pts,
+                        int64_t N, double* out);
+
+/* ---- coefficient fit (ThinShells::cpCoefficients solve step,
+ *      Eigen LeastSquaresConjugateGradient at ThinShells.cpp:615-617) ---- */
+typedef struct {
+    int64_t iterations;       /* inner CGLS iterations (all refinement steps) */
+    int64_t refinements;      /* binary64 residual evaluations after the first */
+    double normal_rel;        /* ||A^T(b - A lambda)|| / ||A^T b||, binary64 */
+    double residual_rel;      /* ||b - A lambda|| / ||b||, binary64 */
+    double residual_abs;      /* ||A lambda - b|| (printed by the reference) */
+    double initial_normal_rel;/* same quantity at the starting guess */
+    double seconds;           /* wall time of the solve on the device stream */
+    int converged;
+} its_lscg_stats;
+/* b: host RHS (NULL = use the device RHS from its_sdf_at_vertices).
+ * lambda_inout: host n-vector; read as the initial guess when warm != 0,
+ * otherwise the solve starts from 0 (Eigen's solve()); always written.
+ * tol: stopping tolerance on normal_rel; max_iter <= 0 means 2n (Eigen's
+ * default).  The solution also becomes the octree's coefficient field. */
+int its_lscg_solve(its_ctx ctx, its_svo svo, const double* b, double* lambda_inout, double tol, int64_t max_iter,
+                   int warm, its_lscg_stats* stats);
+
+/* Matrix-free products with the collocation matrix A_ij = B_j(v_i)
+ * (the assembled matrix of ThinShells.cpp:556-605), binary64, host vectors. */
+int its_collocation_mul(its_ctx ctx, its_svo svo, const double* x, double* y, int transpose);
+
+/* ---- coefficient field used by evaluation and marching cubes ---- */
+int its_svo_set_lambda(its_ctx ctx, its_svo svo, const double* lambda);
+int its_svo_get_lambda(its_svo svo, double* lambda);
+
+/* ---- evaluation of f(x) = sum_i lambda_i B_i(x) ----
+ * mode ITS_EVAL_SUPPORT: the CPU query path mt_cpuTest_2
+ *   (ThinShells.cpp:1249-1294): points outside the open box (origin, end)
+ *   get out_val, others the octree-support sum.
+ * mode ITS_EVAL_DENSE: cuAcc::cpPointQuery (CUDACompute.cu:1070-1156):
+ *   points outside [origin - width, end + width] get out_val, others the sum
+ *   over every basis function (equal to the support sum).
+ * mode ITS_EVAL_RAW: the sum over every basis function everywhere
+ *   (cuAcc::cpBSplineVal, CUDACompute.cu:946-1068; marching-cubes corners).
+ * box_end: AABox boxEnd. */
+#define ITS_EVAL_SUPPORT 0
+#define ITS_EVAL_DENSE 1
+#define ITS_EVAL_RAW 2
+int its_bspline_eval(its_ctx ctx, its_svo svo, const float* pts, int64_t N, const double box_end[3], int mode,
+                     float out_val, float* f);
+int its_bspline_eval_f64pts(its_ctx ctx, its_svo svo, const double* pts, int64_t N, const double box_end[3], int mode,
+                            float out_val, float* f);
+/* device SoA coordinates x[N], y[N], z[N]; f[N] */
+int its_bspline_eval_device(its_ctx ctx, its_svo svo, const float* d_x, const float* d_y, const float* d_z, int64_t N,
+                            const double box_end[3], int mode, float out_val, float* d_f, void* stream);
+/* {-1, 0, 1} classification of f (ThinShells.cpp:1412-1417) */
+int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, int8_t* cls);
+
+/* ---- marching cubes (MC::marching_cubes, MarchingCubes.h:136-139,
+ *      MarchingCubes.cu:672-714) ----
+ * Grid of res[0] x res[1] x res[2] voxels over origin + [0, width).  The
+ * triangle soup is returned as float xyz, 3 vertices per triangle, in the
+ * reference order (voxels x-fastest, triangles in table order).
+ * tri_xyz may be NULL to query the count; capacity is in vertices. */
+typedef struct {
+    int64_t num_verts;        /* allTriVertices */
+    int64_t num_valid_voxels; /* nValidVoxels */
+    double seconds_eval;      /* lattice evaluation */
+    double seconds_extract;   /* classify + scan + emit */
+} its_mc_stats;
+int its_mc_extract(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
+                   double iso, float* tri_xyz, int64_t capacity, its_mc_stats* stats);
+/* lattice values f at the (res+1)^3 grid points, x fastest (for checking) */
+int its_mc_lattice(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
+                   float* values);
+
+/* ---- timing helpers on the context stream (hipEvents) ---- */
+int its_timer_start(its_ctx ctx);
+int its_timer_stop(its_ctx ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,515 @@
+// extern "C" boundary of the ITS gfx950 library (declared in include/its_c.h).
+// Every entry point converts exceptions into a status code and records the
+// message for its_last_error(); nothing escapes as an exception or exit().
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "internal.hpp"
+
+namespace its {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace its
+
+using namespace its;
+
+#define ITS_API_BEGIN try {
+#define ITS_API_END                                      \
+    return ITS_OK;                                       \
+    }                                                    \
+    catch (const its::Error& e) {                        \
+        set_last_error(e.what());                        \
+        return e.code;                                   \
+    }                                                    \
+    catch (const std::bad_alloc&) {                      \
+        set_last_error("host allocation failed");        \
+        return ITS_E_NOMEM;                              \
+    }                                                    \
+    catch (const std::exception& e) {                    \
+        set_last_error(e.what());                        \
+        return ITS_E_INVALID;                            \
+    }
+
+namespace {
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        ITS_HIP(hipGetDevice(&prev));
+        if (prev != dev) ITS_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <class T>
+DBuf<T> upload(const T* h, size_t n, hipStream_t st) {
+    DBuf<T> d(n ? n : 1);
+    if (n) ITS_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+    return d;
+}
+
+void check_svo(its_svo s) { ITS_REQUIRE(s != nullptr && s->ctx != nullptr, ITS_E_INVALID, "null octree handle"); }
+}  // namespace
+
+extern "C" {
+
+const char* its_last_error(void) { return g_last_error.c_str(); }
+const char* its_version(void) { return "its_amd 0.1 (gfx950)"; }
+
+int its_device_count(int* count) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(count, ITS_E_INVALID, "null pointer");
+    ITS_HIP(hipGetDeviceCount(count));
+    ITS_API_END
+}
+
+int its_ctx_create(int device, its_ctx* out) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(out, ITS_E_INVALID, "null output pointer");
+    int n = 0;
+    ITS_HIP(hipGetDeviceCount(&n));
+    ITS_REQUIRE(device >= 0 && device < n, ITS_E_INVALID, "no such device");
+    DeviceGuard g(device);
+    auto* c = new its_ctx_s();
+    c->device = device;
+    ITS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+    ITS_HIP(hipEventCreate(&c->ev0));
+    ITS_HIP(hipEventCreate(&c->ev1));
+    ITS_HIP(hipGetDeviceProperties(&c->prop, device));
+    *out = c;
+    ITS_API_END
+}
+
+int its_ctx_destroy(its_ctx ctx) {
+    ITS_API_BEGIN
+    if (!ctx) return ITS_OK;
+    DeviceGuard g(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+    ITS_API_END
+}
+
+int its_ctx_set_stream(its_ctx ctx, void* hip_stream) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx, ITS_E_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    ITS_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->own_stream) ITS_HIP(hipStreamDestroy(ctx->stream));
+    if (hip_stream) {
+        ctx->stream = (hipStream_t)hip_stream;
+        ctx->own_stream = false;
+    } else {
+        ITS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->own_stream = true;
+    }
+    ITS_API_END
+}
+
+int its_ctx_synchronize(its_ctx ctx) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx, ITS_E_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    ITS_HIP(hipStreamSynchronize(ctx->stream));
+    ITS_API_END
+}
+
+// ---- BaseModel::model2UnitCube (BaseModel.cpp:54-92)
+int its_model_to_unit_cube(double* V, int64_t nV, double scale_factor) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(V && nV > 0, ITS_E_INVALID, "empty vertex array");
+    double bmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = 0; i < nV; ++i)
+        for (int a = 0; a < 3; ++a) {
+            double v = V[3 * i + a];
+            if (v < bmin[a]) bmin[a] = v;
+            if (v > bmax[a]) bmax[a] = v;
+        }
+    double scale = bmax[0] - bmin[0];
+    for (int d = 1; d < 3; ++d) scale = std::max(scale, bmax[d] - bmin[d]);
+    scale *= scale_factor;
+    double center[3];
+    for (int a = 0; a < 3; ++a) center[a] = 0.5 * bmax[a] + 0.5 * bmin[a];
+    for (int a = 0; a < 3; ++a) center[a] -= scale / 2;
+    const double inv = 1. / scale;
+    for (int64_t i = 0; i < nV; ++i)
+        for (int a = 0; a < 3; ++a) V[3 * i + a] = (V[3 * i + a] + (-center[a])) * inv;
+    ITS_API_END
+}
+
+// ---- BaseModel::setUniformBoundingBox (BaseModel.cpp:24-51), float casts kept
+int its_uniform_bbox(const double* V, int64_t nV, double origin[3], double end[3], double width[3]) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(V && nV > 0 && origin && end && width, ITS_E_INVALID, "bad arguments");
+    double minV[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, maxV[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = 0; i < nV; ++i)
+        for (int a = 0; a < 3; ++a) {
+            double v = V[3 * i + a];
+            if (v < minV[a]) minV[a] = v;
+            if (v > maxV[a]) maxV[a] = v;
+        }
+    double lengths[3];
+    for (int a = 0; a < 3; ++a) {
+        origin[a] = minV[a];
+        end[a] = maxV[a];
+        lengths[a] = maxV[a] - minV[a];
+    }
+    float max_length = fmaxf((float)lengths[0], fmaxf((float)lengths[1], (float)lengths[2]));
+    for (int i = 0; i < 3; ++i) {
+        if ((double)max_length == lengths[i]) continue;
+        float delta = (float)((double)max_length - lengths[i]);
+        origin[i] = minV[i] - (double)(delta / 2.0f);
+        end[i] = maxV[i] + (double)(delta / 2.0f);
+    }
+    double eps[3];
+    for (int a = 0; a < 3; ++a) eps[a] = (end[a] - origin[a]) / 11;
+    for (int a = 0; a < 3; ++a) {
+        origin[a] -= eps[a];
+        end[a] += eps[a];
+        width[a] = end[a] - origin[a];
+    }
+    ITS_API_END
+}
+
+// ---- octree
+static int svo_build_common(its_ctx ctx, const double* tris, bool device_ptr, int64_t T, const double origin[3],
+                            const double width[3], int R, its_svo* out) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx && out && origin && width, ITS_E_INVALID, "bad arguments");
+    ITS_REQUIRE(T >= 0 && (T == 0 || tris), ITS_E_INVALID, "bad triangle array");
+    ITS_REQUIRE(R >= 2 && R <= 1024 && (R & (R - 1)) == 0, ITS_E_RANGE, "R must be a power of two in [2, 1024]");
+    DeviceGuard g(ctx->device);
+    std::unique_ptr<its_svo_s> s(new its_svo_s());
+    s->ctx = ctx;
+    s->R = R;
+    for (int a = 0; a < 3; ++a) {
+        s->origin[a] = origin[a];
+        s->width[a] = width[a];
+    }
+    if (device_ptr || T == 0) {
+        svo_build(s.get(), tris, T);
+    } else {
+        DBuf<double> d = upload(tris, 9 * (size_t)T, ctx->stream);
+        svo_build(s.get(), d.p, T);
+    }
+    *out = s.release();
+    ITS_API_END
+}
+
+int its_svo_build(its_ctx ctx, const double* tris, int64_t T, const double origin[3], const double width[3], int R,
+                  its_svo* out) {
+    return svo_build_common(ctx, tris, false, T, origin, width, R, out);
+}
+int its_svo_build_device(its_ctx ctx, const double* d_tris, int64_t T, const double origin[3], const double width[3],
+                         int R, its_svo* out) {
+    return svo_build_common(ctx, d_tris, true, T, origin, width, R, out);
+}
+
+int its_svo_destroy(its_svo svo) {
+    ITS_API_BEGIN
+    if (!svo) return ITS_OK;
+    DeviceGuard g(svo->ctx->device);
+    (void)hipStreamSynchronize(svo->ctx->stream);
+    delete svo;
+    ITS_API_END
+}
+
+int its_svo_get_info(its_svo s, its_svo_info* info) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(info, ITS_E_INVALID, "null info");
+    std::memset(info, 0, sizeof *info);
+    info->tree_depth = s->depth;
+    info->R = s->R;
+    info->num_voxels = s->num_voxels;
+    info->num_nodes = s->num_nodes;
+    info->num_verts = s->num_verts;
+    for (int d = 0; d < s->depth; ++d) info->depth_num_nodes[d] = s->lv[d].n_nodes;
+    for (int d = 0; d <= s->depth; ++d) {
+        info->esum_nodes[d] = s->esum_nodes[d];
+        info->esum_verts[d] = s->esum_verts[d];
+    }
+    for (int a = 0; a < 3; ++a) info->grid_origin[a] = s->origin[a];
+    info->voxel_width = s->voxel_width;
+    ITS_API_END
+}
+
+int its_svo_export_nodes(its_svo s, uint32_t* morton, uint8_t* is_leaf, double* origin, double* width,
+                         uint32_t* parent, uint32_t* childs, uint32_t* neighbors) {
+    ITS_API_BEGIN
+    check_svo(s);
+    DeviceGuard g(s->ctx->device);
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = s->num_nodes;
+    if (n == 0) return ITS_OK;
+    auto cp = [&](void* h, const void* d, size_t bytes) {
+        if (h) ITS_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+    };
+    cp(morton, s->morton.p, n * sizeof(uint32_t));
+    cp(is_leaf, s->is_leaf.p, n);
+    cp(origin, s->node_origin.p, 3 * n * sizeof(double));
+    cp(width, s->node_width.p, n * sizeof(double));
+    cp(parent, s->parent.p, n * sizeof(uint32_t));
+    cp(neighbors, s->neighbors.p, 27 * n * sizeof(uint32_t));
+    if (childs) {
+        std::vector<uint32_t> cb;
+        to_host(cb, s->childbase.p, n, st);
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 8; ++c) childs[8 * i + c] = (cb[i] == ITS_NONE) ? ITS_NONE : cb[i] + (uint32_t)c;
+    }
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+int its_svo_export_verts(its_svo s, double* xyz, uint32_t* owner) {
+    ITS_API_BEGIN
+    check_svo(s);
+    DeviceGuard g(s->ctx->device);
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = s->num_verts;
+    if (n == 0) return ITS_OK;
+    if (xyz) ITS_HIP(hipMemcpyAsync(xyz, s->vxyz.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (owner) ITS_HIP(hipMemcpyAsync(owner, s->vowner.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+int its_svo_export_voxels(its_svo s, uint32_t* morton) {
+    ITS_API_BEGIN
+    check_svo(s);
+    DeviceGuard g(s->ctx->device);
+    if (s->num_voxels && morton) {
+        ITS_HIP(hipMemcpyAsync(morton, s->lv[0].occ.p, s->num_voxels * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               s->ctx->stream));
+        ITS_HIP(hipStreamSynchronize(s->ctx->stream));
+    }
+    ITS_API_END
+}
+
+// ---- right-hand side
+int its_sdf_at_vertices(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* b) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
+    ITS_REQUIRE(V && F && nV > 0 && nF > 0, ITS_E_INVALID, "empty mesh");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    for (int64_t i = 0; i < 3 * nF; ++i)
+        ITS_REQUIRE(F[i] >= 0 && F[i] < nV, ITS_E_INVALID, "face index out of range");
+    DBuf<double> dV = upload(V, 3 * (size_t)nV, st);
+    DBuf<int32_t> dF = upload(F, 3 * (size_t)nF, st);
+    const int64_t n = s->num_verts;
+    s->rhs.ensure(std::max<int64_t>(n, 1));
+    s->rhs_tri.ensure(std::max<int64_t>(n, 1));
+    signed_distance(st, dV.p, nV, dF.p, nF, s->vxyz.p, n, s->rhs.p, s->rhs_tri.p);
+    s->has_rhs = true;
+    if (b && n) {
+        ITS_HIP(hipMemcpyAsync(b, s->rhs.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    ITS_API_END
+}
+
+int its_signed_distance(its_ctx ctx, const double* V, int64_t nV, const int32_t* F, int64_t nF, const double* pts,
+                        int64_t N, double* out) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx && V && F && pts && out && nV > 0 && nF > 0 && N >= 0, ITS_E_INVALID, "bad arguments");
+    for (int64_t i = 0; i < 3 * nF; ++i)
+        ITS_REQUIRE(F[i] >= 0 && F[i] < nV, ITS_E_INVALID, "face index out of range");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (N == 0) return ITS_OK;
+    DBuf<double> dV = upload(V, 3 * (size_t)nV, st);
+    DBuf<int32_t> dF = upload(F, 3 * (size_t)nF, st);
+    DBuf<double> dP = upload(pts, 3 * (size_t)N, st);
+    DBuf<double> dO(N);
+    signed_distance(st, dV.p, nV, dF.p, nF, dP.p, N, dO.p, nullptr);
+    ITS_HIP(hipMemcpyAsync(out, dO.p, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+// ---- fit
+int its_lscg_solve(its_ctx ctx, its_svo s, const double* b, double* lambda_inout, double tol, int64_t max_iter,
+                   int warm, its_lscg_stats* stats) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
+    ITS_REQUIRE(lambda_inout, ITS_E_INVALID, "null lambda");
+    ITS_REQUIRE(tol >= 0, ITS_E_INVALID, "negative tolerance");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int64_t n = s->num_verts;
+    ITS_REQUIRE(n > 0, ITS_E_STATE, "octree has no vertices");
+    DBuf<double> db;
+    const double* d_b;
+    if (b) {
+        db = upload(b, n, st);
+        d_b = db.p;
+    } else {
+        ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "no right-hand side: pass b or call its_sdf_at_vertices");
+        d_b = s->rhs.p;
+    }
+    s->lambda64.ensure(n);
+    if (warm) ITS_HIP(hipMemcpyAsync(s->lambda64.p, lambda_inout, n * sizeof(double), hipMemcpyHostToDevice, st));
+    lscg_solve(s, d_b, s->lambda64.p, tol, max_iter, warm != 0, stats);
+    ITS_HIP(hipMemcpyAsync(lambda_inout, s->lambda64.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    s->has_lambda = true;
+    field_build(s);
+    ITS_API_END
+}
+
+int its_collocation_mul(its_ctx ctx, its_svo s, const double* x, double* y, int transpose) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && x && y, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int64_t n = s->num_verts;
+    DBuf<double> dx = upload(x, n, st), dy(n);
+    collocation_mul(s, dx.p, dy.p, transpose != 0);
+    ITS_HIP(hipMemcpyAsync(y, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+int its_svo_set_lambda(its_ctx ctx, its_svo s, const double* lambda) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && lambda, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    const int64_t n = s->num_verts;
+    s->lambda64.ensure(std::max<int64_t>(n, 1));
+    if (n) ITS_HIP(hipMemcpyAsync(s->lambda64.p, lambda, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    s->has_lambda = true;
+    field_build(s);
+    ITS_API_END
+}
+
+int its_svo_get_lambda(its_svo s, double* lambda) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(lambda, ITS_E_INVALID, "null lambda");
+    ITS_REQUIRE(s->has_lambda, ITS_E_STATE, "no coefficients");
+    DeviceGuard g(s->ctx->device);
+    ITS_HIP(hipMemcpyAsync(lambda, s->lambda64.p, s->num_verts * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+    ITS_HIP(hipStreamSynchronize(s->ctx->stream));
+    ITS_API_END
+}
+
+// ---- evaluation
+static void check_mode(int mode) {
+    ITS_REQUIRE(mode == ITS_EVAL_SUPPORT || mode == ITS_EVAL_DENSE || mode == ITS_EVAL_RAW, ITS_E_INVALID, "bad mode");
+}
+
+int its_bspline_eval(its_ctx ctx, its_svo s, const float* pts, int64_t N, const double box_end[3], int mode,
+                     float out_val, float* f) {
+    ITS_API_BEGIN
+    check_svo(s);
+    check_mode(mode);
+    ITS_REQUIRE(ctx == s->ctx && (N == 0 || (pts && f)) && box_end, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (N == 0) return ITS_OK;
+    std::vector<float> soa(3 * (size_t)N);
+    for (int64_t i = 0; i < N; ++i) {
+        soa[i] = pts[3 * i];
+        soa[N + i] = pts[3 * i + 1];
+        soa[2 * N + i] = pts[3 * i + 2];
+    }
+    DBuf<float> d = upload(soa.data(), soa.size(), st), out(N);
+    eval_points(s, d.p, d.p + N, d.p + 2 * N, N, box_end, mode, out_val, out.p, st);
+    ITS_HIP(hipMemcpyAsync(f, out.p, N * sizeof(float), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+int its_bspline_eval_f64pts(its_ctx ctx, its_svo s, const double* pts, int64_t N, const double box_end[3], int mode,
+                            float out_val, float* f) {
+    ITS_API_BEGIN
+    check_svo(s);
+    check_mode(mode);
+    ITS_REQUIRE(ctx == s->ctx && (N == 0 || (pts && f)) && box_end, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (N == 0) return ITS_OK;
+    DBuf<double> d = upload(pts, 3 * (size_t)N, st);
+    DBuf<float> out(N);
+    eval_points_aos64(s, d.p, N, box_end, mode, out_val, out.p, st);
+    ITS_HIP(hipMemcpyAsync(f, out.p, N * sizeof(float), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+int its_bspline_eval_device(its_ctx ctx, its_svo s, const float* d_x, const float* d_y, const float* d_z, int64_t N,
+                            const double box_end[3], int mode, float out_val, float* d_f, void* stream) {
+    ITS_API_BEGIN
+    check_svo(s);
+    check_mode(mode);
+    ITS_REQUIRE(ctx == s->ctx && box_end && (N == 0 || (d_x && d_y && d_z && d_f)), ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    eval_points(s, d_x, d_y, d_z, N, box_end, mode, out_val, d_f, stream ? (hipStream_t)stream : ctx->stream);
+    ITS_API_END
+}
+
+int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, int8_t* cls) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(N == 0 || (f && cls), ITS_E_INVALID, "bad arguments");
+    for (int64_t i = 0; i < N; ++i) {
+        double v = f[i];
+        cls[i] = (v >= outer_iso) ? 1 : (v <= inner_iso ? -1 : 0);
+    }
+    ITS_API_END
+}
+
+// ---- marching cubes
+int its_mc_extract(its_ctx ctx, its_svo s, const double origin[3], const double width[3], const int res[3], double iso,
+                   float* tri_xyz, int64_t capacity, its_mc_stats* stats) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && origin && width && res, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    mc_extract(s, origin, width, res, iso, tri_xyz, capacity, stats);
+    ITS_API_END
+}
+
+int its_mc_lattice(its_ctx ctx, its_svo s, const double origin[3], const double width[3], const int res[3],
+                   float* values) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && origin && width && res && values, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    mc_lattice(s, origin, width, res, values);
+    ITS_API_END
+}
+
+// ---- timing
+int its_timer_start(its_ctx ctx) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx, ITS_E_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    ITS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    ITS_API_END
+}
+int its_timer_stop(its_ctx ctx, double* ms) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx && ms, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    ITS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ITS_HIP(hipEventSynchronize(ctx->ev1));
+    float t = 0;
+    ITS_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t;
+    ITS_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// Device-wide scans (three-phase: per-block local scan, scan of block sums,
+// add back) and the rank directory of bitsets.  wave64 shuffles do the
+// intra-wave part; one block covers 2048 items.
+#include "common.hpp"
+
+namespace its {
+
+namespace {
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// in -> exclusive prefix within the block tile; block total to sums[block]
+template <class T, class Load, class Store>
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(Load load, Store store, T* sums, int64_t n) {
+    __shared__ T wave_tot[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? load(base + i) : T(0);
+        sum += v[i];
+    }
+    T incl = wave_incl_scan(sum);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wave_tot[wid] = incl;
+    __syncthreads();
+    T woff = 0;
+    for (int w = 0; w < wid; ++w) woff += wave_tot[w];
+    T run = woff + incl - sum;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) store(base + i, run);
+        run += v[i];
+    }
+    if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = run;
+}
+
+template <class T, class Load, class Store>
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(Load load, Store store, const T* offs, int64_t n) {
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
+    const T o = offs[blockIdx.x];
+    if (o == 0) return;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t j = base + (int64_t)i * kScanThreads;
+        if (j < n) store(j, load(j) + o);
+    }
+}
+
+template <class T>
+struct ArrLoad {
+    const T* p;
+    __device__ T operator()(int64_t i) const { return p[i]; }
+};
+template <class T>
+struct ArrStore {
+    T* p;
+    __device__ void operator()(int64_t i, T v) const { p[i] = v; }
+};
+struct PopcLoad {
+    const RankWord* g;
+    __device__ uint32_t operator()(int64_t i) const { return (uint32_t)__popc(g[i].bits); }
+};
+struct RankStore {
+    RankWord* g;
+    __device__ void operator()(int64_t i, uint32_t v) const { g[i].rank = v; }
+};
+struct RankLoad {
+    const RankWord* g;
+    __device__ uint32_t operator()(int64_t i) const { return g[i].rank; }
+};
+
+template <class T, class Load, class Store, class AddLoad>
+uint64_t scan_impl(Load load, Store store, AddLoad addload, int64_t n, hipStream_t s) {
+    if (n <= 0) return 0;
+    int64_t nb = (n + kScanTile - 1) / kScanTile;
+    DBuf<T> sums(nb);
+    hipLaunchKernelGGL((k_scan_local<T, Load, Store>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, load, store, sums.p,
+                       n);
+    ITS_LAUNCH_CHECK();
+    uint64_t total;
+    if (nb == 1) {
+        total = (uint64_t)read1(sums.p, s);
+    } else {
+        total = scan_impl<T>(ArrLoad<T>{sums.p}, ArrStore<T>{sums.p}, ArrLoad<T>{sums.p}, nb, s);
+        hipLaunchKernelGGL((k_scan_add<T, AddLoad, Store>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, addload, store,
+                           sums.p, n);
+        ITS_LAUNCH_CHECK();
+    }
+    ITS_HIP(hipStreamSynchronize(s));
+    return total;
+}
+}  // namespace
+
+uint64_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s) {
+    return scan_impl<uint32_t>(ArrLoad<uint32_t>{in}, ArrStore<uint32_t>{out}, ArrLoad<uint32_t>{out}, n, s);
+}
+uint64_t scan_exclusive_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s) {
+    return scan_impl<uint64_t>(ArrLoad<uint64_t>{in}, ArrStore<uint64_t>{out}, ArrLoad<uint64_t>{out}, n, s);
+}
+uint64_t rank_build(RankWord* g, int64_t nwords, hipStream_t s) {
+    return scan_impl<uint32_t>(PopcLoad{g}, RankStore{g}, RankLoad{g}, nwords, s);
+}
+
+}  // namespace its

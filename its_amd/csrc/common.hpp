@@ -1,0 +1,171 @@
+// Shared device infrastructure of the ITS gfx950 library: error plumbing,
+// device buffers, Morton bit tricks, wave64 / block / device-wide scans.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "its_c.h"
+
+namespace its {
+
+// ---------------------------------------------------------------- errors ----
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void set_last_error(const std::string& msg);
+
+#define ITS_HIP(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::its::Error(ITS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                                              __FILE__ + ":" + std::to_string(__LINE__));               \
+    } while (0)
+#define ITS_REQUIRE(cond, code, msg)                      \
+    do {                                                  \
+        if (!(cond)) throw ::its::Error((code), (msg));   \
+    } while (0)
+#define ITS_LAUNCH_CHECK() ITS_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- buffers ---
+// Device allocation owned by a host object.  Allocations go through
+// hipMallocAsync-free plain hipMalloc so buffers outlive streams.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) {
+            hipError_t e = hipMalloc(&p, count * sizeof(T));
+            if (e != hipSuccess) {
+                p = nullptr;
+                n = 0;
+                throw Error(ITS_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                                             hipGetErrorString(e));
+            }
+        }
+    }
+    // grow-only reuse
+    void ensure(size_t count) { if (count > n) alloc(count); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+inline unsigned blocks_for(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+// ---------------------------------------------------------------- Morton ----
+// x -> bit 3i, y -> bit 3i+1, z -> bit 3i+2 (coordinates < 1024)
+__host__ __device__ inline uint32_t spread3(uint32_t x) {
+    x &= 0x3FF;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__host__ __device__ inline uint32_t compact3(uint32_t x) {
+    x &= 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030C30C3u;
+    x = (x ^ (x >> 4)) & 0x0300F00Fu;
+    x = (x ^ (x >> 8)) & 0x030000FFu;
+    x = (x ^ (x >> 16)) & 0x000003FFu;
+    return x;
+}
+__host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+__host__ __device__ inline void demorton3(uint32_t m, uint32_t& x, uint32_t& y, uint32_t& z) {
+    x = compact3(m);
+    y = compact3(m >> 1);
+    z = compact3(m >> 2);
+}
+
+// ---------------------------------------------------------------- bitsets ---
+// A bitset with a rank directory: word w holds bits [32w, 32w+32) and the
+// number of set bits in all words before it.
+struct RankWord {
+    uint32_t bits;
+    uint32_t rank;
+};
+
+__device__ inline bool bit_test(const RankWord* g, uint32_t i) { return (g[i >> 5].bits >> (i & 31)) & 1u; }
+// index of set bit i among all set bits; caller ensures the bit is set
+__device__ inline uint32_t bit_rank(const RankWord* g, uint32_t i) {
+    RankWord w = g[i >> 5];
+    return w.rank + __popc(w.bits & ((1u << (i & 31)) - 1u));
+}
+// returns ITS_NONE if not set
+__device__ inline uint32_t bit_find(const RankWord* g, uint32_t i) {
+    RankWord w = g[i >> 5];
+    uint32_t b = 1u << (i & 31);
+    if (!(w.bits & b)) return ITS_NONE;
+    return w.rank + __popc(w.bits & (b - 1u));
+}
+
+// ---------------------------------------------------------------- scans -----
+// wave64 inclusive scan of unsigned values
+template <class T>
+__device__ inline T wave_incl_scan(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Device-wide exclusive scan of n values of T (uint32_t or uint64_t).
+// out may alias in.  Returns the total (synchronises the stream).
+uint64_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s);
+uint64_t scan_exclusive_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s);
+// Fill the rank fields of a RankWord array from its bits; returns the total.
+uint64_t rank_build(RankWord* g, int64_t nwords, hipStream_t s);
+
+template <class T>
+void to_host(std::vector<T>& h, const T* d, size_t n, hipStream_t s) {
+    h.resize(n);
+    if (n) ITS_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    ITS_HIP(hipStreamSynchronize(s));
+}
+template <class T>
+T read1(const T* d, hipStream_t s) {
+    T v;
+    ITS_HIP(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, s));
+    ITS_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+}  // namespace its

@@ -1,0 +1,94 @@
+// Internal object layout of the ITS gfx950 library (not part of the C ABI).
+#pragma once
+#include <memory>
+
+#include "common.hpp"
+
+namespace its {
+
+constexpr int kMaxLevels = ITS_MAX_LEVELS;
+
+// One octree level (0 = finest voxels).
+struct Level {
+    int64_t n_nodes = 0;    // nodes (octets of children of occupied parents)
+    int64_t n_occ = 0;      // occupied cells (sorted Morton codes)
+    int64_t n_verts = 0;    // distinct node corners
+    int R = 1;              // cells per axis at this level
+    double width = 0;       // node width
+    DBuf<uint32_t> occ;     // occupied cell codes, ascending
+    DBuf<uint32_t> prank;   // rank of each occupied cell's parent among the parents
+    DBuf<RankWord> lattice; // corner lattice (R+1)^3 bits, index (x*(R+1)+y)*(R+1)+z
+    // evaluation field: cells whose corners include a vertex
+    DBuf<RankWord> cells;   // R^3 bits by Morton code
+    DBuf<float4> blocks;    // 2 float4 (8 corner coefficients) per marked cell
+    int64_t n_cells = 0;
+};
+
+struct Fit;   // matrix-free collocation operator and solver state (lscg.hip)
+struct McScratch;
+
+}  // namespace its
+
+struct its_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipDeviceProp_t prop{};
+};
+
+struct its_svo_s {
+    its_ctx ctx = nullptr;
+    int depth = 0;
+    int R = 0;
+    double origin[3] = {0, 0, 0};
+    double width[3] = {0, 0, 0};
+    double voxel_width = 0;
+    int64_t num_voxels = 0, num_nodes = 0, num_verts = 0;
+    int64_t esum_nodes[its::kMaxLevels + 1] = {0};
+    int64_t esum_verts[its::kMaxLevels + 1] = {0};
+    its::Level lv[its::kMaxLevels];
+
+    // nodes, global order finest -> root
+    its::DBuf<uint32_t> morton, parent, childbase, neighbors, node_corner;
+    its::DBuf<uint8_t> is_leaf;
+    its::DBuf<double> node_origin, node_width;
+    // vertices, global order level by level, lexicographic within a level
+    its::DBuf<double> vxyz;
+    its::DBuf<uint32_t> vowner;
+    its::DBuf<uint16_t> vx, vy, vz;   // lattice coordinates at the vertex's level
+    its::DBuf<uint8_t> vlevel;
+
+    // right-hand side and coefficient field
+    its::DBuf<double> rhs;
+    its::DBuf<int32_t> rhs_tri;     // nearest triangle per vertex
+    bool has_rhs = false;
+    its::DBuf<double> lambda64;
+    bool has_lambda = false;
+    bool field_ready = false;
+
+    std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};
+};
+
+namespace its {
+// svo.hip
+void svo_build(its_svo s, const double* d_tris, int64_t T);
+// eval.hip
+void field_build(its_svo s);   // cells + blocks from lambda64
+void eval_points(its_svo s, const float* x, const float* y, const float* z, int64_t N, const double box_end[3],
+                 int mode, float out_val, float* f, hipStream_t st);
+void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box_end[3], int mode, float out_val,
+                       float* f, hipStream_t st);
+// lscg.hip
+void fit_release(Fit* f);
+void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int64_t max_iter, bool warm,
+                its_lscg_stats* st);
+void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose);
+// mc.hip
+void mc_extract(its_svo s, const double origin[3], const double width[3], const int res[3], double iso,
+                float* h_tri, int64_t capacity, its_mc_stats* st);
+void mc_lattice(its_svo s, const double origin[3], const double width[3], const int res[3], float* h_vals);
+// sdf.hip
+void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_t* d_F, int64_t nF,
+                     const double* d_pts, int64_t N, double* d_out, int32_t* d_tri);
+}  // namespace its

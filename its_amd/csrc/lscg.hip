@@ -1,0 +1,667 @@
+// Least-squares conjugate gradient fit of the coefficients for gfx950.
+//
+// Replaces the assembly and Eigen solve of ThinShells::cpCoefficients
+// (ThinShells.cpp:511-617).  The reference assembles the n x n collocation
+// matrix A_ij = B_j(v_i) from triplets and calls
+// Eigen::LeastSquaresConjugateGradient with its defaults.  That is CGLS with a
+// diagonal preconditioner 1/||A_:j||^2, starting from x = 0, and the stop test
+// ||A^T(b - Ax)||^2 < tol^2 ||A^T b||^2.
+//
+// Here A is never assembled.  The basis is hierarchical degree-1 B-splines, so
+// the two-scale relation gives A = D G:
+//   - G = (I - P)^-1.  It maps coefficients to g, the value of the
+//     coarser-or-equal part of f at every vertex.  P prolongates level d+1 to
+//     level d by trilinear interpolation.  A vertex with k odd lattice
+//     coordinates averages 2^k coarse vertices with weight 2^-k.
+//   - D picks, for every vertex, g at the finest vertex at the same position.
+//     Finer coincident basis functions are 1 there and the rest vanish.
+// A p is one top-down pass over the levels.  A^T r is the transposed
+// bottom-up pass: r summed along each coincidence chain, then restricted
+// level by level.  Per iteration this moves about 2 x (4 + 4 + 3.4 x 4) bytes
+// per vertex, against 16 bytes per non-zero (about 50 per row) for the
+// assembled matrix.
+//
+// Precision: the inner CGLS keeps its vectors and matvec in binary32 and
+// accumulates every dot product in binary64 (products of binary32 values are
+// exact in binary64).  An outer loop recomputes the true residual b - A lambda
+// and the normal residual in binary64.  It refines lambda in binary64 until
+// the binary64 normal residual meets the tolerance.
+// All reductions are per-block partials summed in fixed order, so results are
+// deterministic.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace its {
+
+struct Fit {
+    int64_t n = 0;
+    int L = 0;
+    int64_t off[kMaxLevels + 1] = {0};
+    DBuf<uint32_t> pro_ptr, pro_idx;   // prolongation: coarse parents of each vertex
+    DBuf<uint32_t> res_ptr, res_idx;   // restriction: fine children of each vertex (| log2 count << 28)
+    DBuf<uint32_t> down, up;           // finest coincident vertex; coincident vertex one level up
+    DBuf<uint8_t> finest;
+    DBuf<float> invd32;
+    DBuf<double> invd64;
+    // inner CGLS vectors (binary32)
+    DBuf<float> x, r, p, z, t, g, h;
+    // binary64 work vectors
+    DBuf<double> g64, h64, r64, nr64, b64, tmp64;
+    // reductions and device scalars
+    DBuf<double> part;
+    DBuf<double> scal;     // see Scal
+    DBuf<uint32_t> ticket;
+    DBuf<int> ctl;         // [0] done, [1] iterations, [2] structure error
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_chunk = 0;
+    ~Fit() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+void fit_release(Fit* f) { delete f; }
+
+namespace {
+constexpr int kT = 256;
+constexpr int kRedBlocks = 512;  // fixed partial count for deterministic sums
+
+// device scalar slots
+enum Scal { S_ABSNEW = 0, S_ALPHA, S_BETA, S_TT, S_RN2, S_THRESH, S_RN2_0, S_ZZ, S_COUNT };
+
+// ---------------------------------------------------------------- structure
+__global__ void k_pro_count(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
+                            const uint16_t* vz, uint32_t* cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (lev[i] == L - 1) { cnt[i] = 0; return; }
+    int k = (vx[i] & 1) + (vy[i] & 1) + (vz[i] & 1);
+    cnt[i] = 1u << k;
+}
+
+struct LatticeSet {
+    const RankWord* lat[kMaxLevels];
+    uint32_t D[kMaxLevels];
+    uint32_t off[kMaxLevels];
+};
+
+__device__ inline uint32_t lat_find(const LatticeSet& S, int d, uint32_t X, uint32_t Y, uint32_t Z) {
+    uint32_t D = S.D[d];
+    if (X >= D || Y >= D || Z >= D) return ITS_NONE;
+    uint32_t r = bit_find(S.lat[d], (X * D + Y) * D + Z);
+    return r == ITS_NONE ? ITS_NONE : S.off[d] + r;
+}
+
+__global__ void k_pro_fill(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
+                           const uint16_t* vz, LatticeSet S, const uint32_t* ptr, uint32_t* idx, int* err) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = lev[i];
+    if (d == L - 1) return;
+    uint32_t X = vx[i], Y = vy[i], Z = vz[i];
+    uint32_t xs[2] = {X >> 1, (X + 1) >> 1}, ys[2] = {Y >> 1, (Y + 1) >> 1}, zs[2] = {Z >> 1, (Z + 1) >> 1};
+    int nx = (X & 1) ? 2 : 1, ny = (Y & 1) ? 2 : 1, nz = (Z & 1) ? 2 : 1;
+    uint32_t o = ptr[i];
+    for (int c = 0; c < nz; ++c)
+        for (int b = 0; b < ny; ++b)
+            for (int a = 0; a < nx; ++a) {
+                uint32_t j = lat_find(S, d + 1, xs[a], ys[b], zs[c]);
+                if (j == ITS_NONE) { atomicOr(err, 1); j = (uint32_t)i; }
+                idx[o++] = j;
+            }
+}
+
+__global__ void k_res_count(int64_t n, const uint32_t* ptr, const uint32_t* idx, uint32_t* cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t k = ptr[i]; k < ptr[i + 1]; ++k) atomicAdd(&cnt[idx[k]], 1u);
+}
+__global__ void k_res_fill(int64_t n, const uint32_t* ptr, const uint32_t* idx, uint32_t* cursor, uint32_t* ridx) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = ptr[i], e = ptr[i + 1];
+    uint32_t lg = __ffs(e - b) - 1;   // log2 of the count
+    for (uint32_t k = b; k < e; ++k) {
+        uint32_t pos = atomicAdd(&cursor[idx[k]], 1u);
+        ridx[pos] = (uint32_t)i | (lg << 28);
+    }
+}
+// deterministic order inside each restriction row
+__global__ void k_res_sort(int64_t n, const uint32_t* ptr, uint32_t* ridx) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = ptr[i], e = ptr[i + 1];
+    for (uint32_t k = b + 1; k < e; ++k) {
+        uint32_t v = ridx[k];
+        uint32_t key = v & 0x0FFFFFFFu;
+        uint32_t j = k;
+        while (j > b && (ridx[j - 1] & 0x0FFFFFFFu) > key) { ridx[j] = ridx[j - 1]; --j; }
+        ridx[j] = v;
+    }
+}
+
+__global__ void k_updown(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
+                         const uint16_t* vz, LatticeSet S, uint32_t* up, uint32_t* dn1) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = lev[i];
+    uint32_t X = vx[i], Y = vy[i], Z = vz[i];
+    uint32_t u = ITS_NONE, w = ITS_NONE;
+    if (d < L - 1 && !(X & 1) && !(Y & 1) && !(Z & 1)) u = lat_find(S, d + 1, X >> 1, Y >> 1, Z >> 1);
+    if (d > 0) w = lat_find(S, d - 1, 2 * X, 2 * Y, 2 * Z);
+    up[i] = u;
+    dn1[i] = w;
+}
+__global__ void k_down(int64_t n, const uint32_t* dn1, uint32_t* down, uint8_t* finest) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = (uint32_t)i;
+    while (dn1[k] != ITS_NONE) k = dn1[k];
+    down[i] = k;
+    finest[i] = (k == (uint32_t)i);
+}
+
+// Row i of A: 1 at itself and at every finer coincident vertex, and the hat
+// values of the corners of the cell containing v_i at every coarser level
+// (the triplets of ThinShells.cpp:567-601).  Squares accumulate per column
+// in 2^-32 fixed point, so the sum is exact integer arithmetic and the result
+// is independent of the order of the atomics.
+__global__ void k_colnorm(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
+                          const uint16_t* vz, LatticeSet S, const uint32_t* dn1, unsigned long long* acc, int* err) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long one = 1ull << 32;
+    atomicAdd(&acc[i], one);
+    for (uint32_t k = dn1[i]; k != ITS_NONE; k = dn1[k]) atomicAdd(&acc[k], one);
+    int d = lev[i];
+    uint32_t X = vx[i], Y = vy[i], Z = vz[i];
+    for (int l = d + 1; l < L; ++l) {
+        int s = l - d;
+        uint32_t mask = (1u << s) - 1u;
+        double inv = 1.0 / (double)(1u << s);
+        uint32_t q[3] = {X >> s, Y >> s, Z >> s};
+        uint32_t rm[3] = {X & mask, Y & mask, Z & mask};
+        double hv[3][2];
+        int cnt[3];
+        for (int a = 0; a < 3; ++a) {
+            double fr = (double)rm[a] * inv;
+            hv[a][0] = 1.0 - fr;
+            hv[a][1] = fr;
+            cnt[a] = rm[a] ? 2 : 1;
+        }
+        for (int c = 0; c < cnt[2]; ++c)
+            for (int b = 0; b < cnt[1]; ++b)
+                for (int a = 0; a < cnt[0]; ++a) {
+                    uint32_t j = lat_find(S, l, q[0] + a, q[1] + b, q[2] + c);
+                    if (j == ITS_NONE) { atomicOr(err, 2); continue; }
+                    double v = hv[0][a] * hv[1][b] * hv[2][c];
+                    atomicAdd(&acc[j], (unsigned long long)llrint(v * v * 4294967296.0));
+                }
+    }
+}
+__global__ void k_invdiag(int64_t n, const unsigned long long* acc, double* d64, float* d32) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = (double)acc[i] * (1.0 / 4294967296.0);
+    double v = s > 0.0 ? 1.0 / s : 1.0;
+    d64[i] = v;
+    d32[i] = (float)v;
+}
+
+// ---------------------------------------------------------------- operator
+// g = G p on one level: g_i = p_i + 2^-k sum_{c in pro(i)} g_c.  When upd is
+// set the first pass also forms the new search direction p = z + beta p.
+template <class T, bool UPD>
+__global__ __launch_bounds__(kT) void k_prolong(int64_t b, int64_t e, const uint32_t* ptr, const uint32_t* idx, T* p,
+                                               const T* z, T* g, const double* scal, const int* ctl) {
+    if (UPD && ctl[0]) return;
+    int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    T pi = p[i];
+    if (UPD) {
+        pi = z[i] + (T)scal[S_BETA] * pi;
+        p[i] = pi;
+    }
+    uint32_t k0 = ptr[i], k1 = ptr[i + 1];
+    T s = 0;
+    for (uint32_t k = k0; k < k1; ++k) s += g[idx[k]];
+    T w = (k1 > k0) ? (T)1 / (T)(k1 - k0) : (T)0;
+    g[i] = pi + w * s;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
+// last-block-done reduction: returns true in the one block that sees all
+// partials; that block sums them in index order.
+__device__ inline bool last_block(uint32_t* ticket) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+    __syncthreads();
+    return is_last;
+}
+
+// t = D g, ||t||^2, then alpha = absNew / ||t||^2
+__global__ __launch_bounds__(kT) void k_apply_down(int64_t n, const uint32_t* down, const float* g, float* t,
+                                                  double* part, uint32_t* ticket, double* scal, const int* ctl) {
+    if (ctl[0]) return;
+    __shared__ double sh[kT / 64];
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = g[down[i]];
+        t[i] = v;
+        acc += (double)v * (double)v;
+    }
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (last_block(ticket)) {
+        if (threadIdx.x == 0) {
+            double tt = 0;
+            for (unsigned b = 0; b < gridDim.x; ++b) tt += part[b];
+            scal[S_TT] = tt;
+            scal[S_ALPHA] = tt > 0 ? scal[S_ABSNEW] / tt : 0.0;
+            *ticket = 0;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha t along each coincidence chain; h = D^T r
+__global__ __launch_bounds__(kT) void k_update_dt(int64_t n, const uint8_t* finest, const uint32_t* up, float* x,
+                                                 const float* p, float* r, const float* t, float* h, const double* scal,
+                                                 const int* ctl, int do_update) {
+    if (ctl[0]) return;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float alpha = do_update ? (float)scal[S_ALPHA] : 0.f;
+    if (do_update) x[i] = fmaf(alpha, p[i], x[i]);
+    if (!finest[i]) { h[i] = 0.f; return; }
+    float s = 0.f;
+    for (uint32_t j = (uint32_t)i; j != ITS_NONE; j = up[j]) {
+        float rj = r[j];
+        if (do_update) { rj = fmaf(-alpha, t[j], rj); r[j] = rj; }
+        s += rj;
+    }
+    h[i] = s;
+}
+
+// h_c += sum_{y in res(c)} 2^-k(y) h_y for the vertices of one level
+template <class T>
+__global__ __launch_bounds__(kT) void k_restrict(int64_t b, int64_t e, const uint32_t* ptr, const uint32_t* ridx, T* h,
+                                                const int* ctl, int check) {
+    if (check && ctl[0]) return;
+    int64_t c = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= e) return;
+    T s = 0;
+    for (uint32_t k = ptr[c]; k < ptr[c + 1]; ++k) {
+        uint32_t v = ridx[k];
+        T w = (T)1 / (T)(1u << (v >> 28));
+        s += w * h[v & 0x0FFFFFFFu];
+    }
+    h[c] += s;
+}
+
+// z = M^-1 nr; rn2 = nr.nr; absNew' = nr.z; convergence test; beta
+__global__ __launch_bounds__(kT) void k_precond(int64_t n, const float* nr, const float* invd, float* z, double* part,
+                                               uint32_t* ticket, double* scal, int* ctl, int init) {
+    if (ctl[0]) return;
+    __shared__ double sh[kT / 64];
+    double a = 0, c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = nr[i];
+        float zi = invd[i] * v;
+        z[i] = zi;
+        a += (double)v * (double)v;
+        c += (double)v * (double)zi;
+    }
+    double sa = block_sum(a, sh);
+    double sc = block_sum(c, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sa;
+        part[2 * blockIdx.x + 1] = sc;
+    }
+    if (last_block(ticket)) {
+        if (threadIdx.x == 0) {
+            double rn2 = 0, zz = 0;
+            for (unsigned b = 0; b < gridDim.x; ++b) {
+                rn2 += part[2 * b];
+                zz += part[2 * b + 1];
+            }
+            scal[S_RN2] = rn2;
+            if (init) {
+                scal[S_RN2_0] = rn2;
+                scal[S_ABSNEW] = zz;
+                scal[S_BETA] = 0.0;
+                if (rn2 < scal[S_THRESH] || zz == 0.0) ctl[0] = 1;
+            } else {
+                ctl[1] += 1;
+                if (rn2 < scal[S_THRESH] || ctl[1] >= ctl[3]) {
+                    ctl[0] = 1;
+                } else {
+                    double absOld = scal[S_ABSNEW];
+                    scal[S_ABSNEW] = zz;
+                    scal[S_BETA] = zz / absOld;
+                }
+            }
+            *ticket = 0;
+        }
+    }
+}
+
+// binary64 helpers for the refinement loop
+__global__ void k_apply_down64(int64_t n, const uint32_t* down, const double* g, double* t) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) t[i] = g[down[i]];
+}
+__global__ void k_dt64(int64_t n, const uint8_t* finest, const uint32_t* up, const double* r, double* h) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!finest[i]) { h[i] = 0.0; return; }
+    double s = 0.0;
+    for (uint32_t j = (uint32_t)i; j != ITS_NONE; j = up[j]) s += r[j];
+    h[i] = s;
+}
+__global__ void k_sub64(int64_t n, const double* b, const double* a, double* r) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - a[i];
+}
+__global__ void k_to32(int64_t n, const double* a, float* b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_axpy_f32_to64(int64_t n, const float* x, double* y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += (double)x[i];
+}
+__global__ __launch_bounds__(kT) void k_dot64(int64_t n, const double* a, const double* b, double* part,
+                                             uint32_t* ticket, double* out) {
+    __shared__ double sh[kT / 64];
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += a[i] * b[i];
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (last_block(ticket)) {
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (unsigned k = 0; k < gridDim.x; ++k) t += part[k];
+            *out = t;
+            *ticket = 0;
+        }
+    }
+}
+__global__ void k_set_thresh(double* scal, double rel2) { scal[S_THRESH] = rel2 * scal[S_RN2_0]; }
+
+// ---------------------------------------------------------------- host side
+unsigned red_blocks(int64_t n) { return (unsigned)std::min<int64_t>(kRedBlocks, blocks_for(n, kT)); }
+
+Fit* fit_get(its_svo s) {
+    if (s->fit) return s->fit.get();
+    hipStream_t st = s->ctx->stream;
+    auto f = std::unique_ptr<Fit>(new Fit());
+    const int64_t n = s->num_verts;
+    const int L = s->depth;
+    f->n = n;
+    f->L = L;
+    for (int d = 0; d <= L; ++d) f->off[d] = s->esum_verts[d];
+    LatticeSet S{};
+    for (int d = 0; d < L; ++d) {
+        S.lat[d] = s->lv[d].lattice.p;
+        S.D[d] = (uint32_t)s->lv[d].R + 1;
+        S.off[d] = (uint32_t)s->esum_verts[d];
+    }
+    f->ctl.alloc(4);
+    ITS_HIP(hipMemsetAsync(f->ctl.p, 0, 4 * sizeof(int), st));
+    int* err = f->ctl.p + 2;
+    // prolongation lists
+    f->pro_ptr.alloc(n + 1);
+    hipLaunchKernelGGL(k_pro_count, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
+                       s->vz.p, f->pro_ptr.p);
+    ITS_LAUNCH_CHECK();
+    int64_t npro = (int64_t)scan_exclusive_u32(f->pro_ptr.p, f->pro_ptr.p, n, st);
+    uint32_t npro32 = (uint32_t)npro;
+    ITS_HIP(hipMemcpyAsync(f->pro_ptr.p + n, &npro32, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    f->pro_idx.alloc(std::max<int64_t>(npro, 1));
+    hipLaunchKernelGGL(k_pro_fill, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
+                       s->vz.p, S, f->pro_ptr.p, f->pro_idx.p, err);
+    ITS_LAUNCH_CHECK();
+    // restriction lists (transpose of prolongation)
+    f->res_ptr.alloc(n + 1);
+    ITS_HIP(hipMemsetAsync(f->res_ptr.p, 0, (n + 1) * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_res_count, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->pro_ptr.p, f->pro_idx.p,
+                       f->res_ptr.p);
+    ITS_LAUNCH_CHECK();
+    int64_t nres = (int64_t)scan_exclusive_u32(f->res_ptr.p, f->res_ptr.p, n, st);
+    ITS_REQUIRE(nres == npro, ITS_E_STATE, "restriction size mismatch");
+    ITS_HIP(hipMemcpyAsync(f->res_ptr.p + n, &npro32, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    f->res_idx.alloc(std::max<int64_t>(nres, 1));
+    {
+        DBuf<uint32_t> cursor(n + 1);
+        ITS_HIP(hipMemcpyAsync(cursor.p, f->res_ptr.p, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_res_fill, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->pro_ptr.p, f->pro_idx.p,
+                           cursor.p, f->res_idx.p);
+        ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_res_sort, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->res_ptr.p, f->res_idx.p);
+        ITS_LAUNCH_CHECK();
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    // coincidence chains
+    f->up.alloc(n);
+    f->down.alloc(n);
+    f->finest.alloc(n);
+    {
+        DBuf<uint32_t> dn1(n);
+        hipLaunchKernelGGL(k_updown, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
+                           s->vz.p, S, f->up.p, dn1.p);
+        ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_down, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, dn1.p, f->down.p, f->finest.p);
+        ITS_LAUNCH_CHECK();
+        // preconditioner: inverse squared column norms
+        DBuf<unsigned long long> acc(n);
+        ITS_HIP(hipMemsetAsync(acc.p, 0, acc.bytes(), st));
+        hipLaunchKernelGGL(k_colnorm, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
+                           s->vz.p, S, dn1.p, acc.p, err);
+        ITS_LAUNCH_CHECK();
+        f->invd64.alloc(n);
+        f->invd32.alloc(n);
+        hipLaunchKernelGGL(k_invdiag, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, acc.p, f->invd64.p, f->invd32.p);
+        ITS_LAUNCH_CHECK();
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    int e = read1(err, st);
+    ITS_REQUIRE(e == 0, ITS_E_STATE, "octree lattice is inconsistent (missing coarse corner vertex)");
+    for (auto* v : {&f->x, &f->r, &f->p, &f->z, &f->t, &f->g, &f->h}) v->alloc(n);
+    for (auto* v : {&f->g64, &f->h64, &f->r64, &f->nr64, &f->b64, &f->tmp64}) v->alloc(n);
+    f->part.alloc(2 * kRedBlocks);
+    f->scal.alloc(S_COUNT);
+    f->ticket.alloc(1);
+    ITS_HIP(hipMemsetAsync(f->ticket.p, 0, sizeof(uint32_t), st));
+    ITS_HIP(hipMemsetAsync(f->scal.p, 0, S_COUNT * sizeof(double), st));
+    s->fit = std::unique_ptr<Fit, void (*)(Fit*)>(f.release(), fit_release);
+    return s->fit.get();
+}
+
+// y = A x (binary64), via g64
+void apply_A64(Fit* f, const double* x, double* y, hipStream_t st) {
+    double* g = f->g64.p;
+    for (int d = f->L - 1; d >= 0; --d) {
+        int64_t b = f->off[d], e = f->off[d + 1];
+        if (e <= b) continue;
+        hipLaunchKernelGGL((k_prolong<double, false>), dim3(blocks_for(e - b, kT)), dim3(kT), 0, st, b, e, f->pro_ptr.p,
+                           f->pro_idx.p, const_cast<double*>(x), (const double*)nullptr, g, (const double*)nullptr,
+                           (const int*)nullptr);
+        ITS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_apply_down64, dim3(blocks_for(f->n, kT)), dim3(kT), 0, st, f->n, f->down.p, g, y);
+    ITS_LAUNCH_CHECK();
+}
+// y = A^T x (binary64), y must not alias x
+void apply_AT64(Fit* f, const double* x, double* y, hipStream_t st) {
+    hipLaunchKernelGGL(k_dt64, dim3(blocks_for(f->n, kT)), dim3(kT), 0, st, f->n, f->finest.p, f->up.p, x, y);
+    ITS_LAUNCH_CHECK();
+    for (int d = 1; d < f->L; ++d) {
+        int64_t b = f->off[d], e = f->off[d + 1];
+        if (e <= b) continue;
+        hipLaunchKernelGGL((k_restrict<double>), dim3(blocks_for(e - b, kT)), dim3(kT), 0, st, b, e, f->res_ptr.p,
+                           f->res_idx.p, y, (const int*)nullptr, 0);
+        ITS_LAUNCH_CHECK();
+    }
+}
+double dot64(Fit* f, const double* a, const double* b, hipStream_t st) {
+    unsigned nb = red_blocks(f->n);
+    hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kT), 0, st, f->n, a, b, f->part.p, f->ticket.p, f->scal.p + S_ZZ);
+    ITS_LAUNCH_CHECK();
+    return read1(f->scal.p + S_ZZ, st);
+}
+
+// binary32 transposed pass on f->r into f->h (with optional fused update)
+void launch_AT32(Fit* f, hipStream_t st, int do_update, int check) {
+    hipLaunchKernelGGL(k_update_dt, dim3(blocks_for(f->n, kT)), dim3(kT), 0, st, f->n, f->finest.p, f->up.p, f->x.p,
+                       f->p.p, f->r.p, f->t.p, f->h.p, f->scal.p, f->ctl.p, do_update);
+    ITS_LAUNCH_CHECK();
+    for (int d = 1; d < f->L; ++d) {
+        int64_t b = f->off[d], e = f->off[d + 1];
+        if (e <= b) continue;
+        hipLaunchKernelGGL((k_restrict<float>), dim3(blocks_for(e - b, kT)), dim3(kT), 0, st, b, e, f->res_ptr.p,
+                           f->res_idx.p, f->h.p, f->ctl.p, check);
+        ITS_LAUNCH_CHECK();
+    }
+}
+
+// one CGLS iteration (all kernels early-out once ctl[0] is set)
+void launch_iteration(Fit* f, hipStream_t st) {
+    const int64_t n = f->n;
+    for (int d = f->L - 1; d >= 0; --d) {
+        int64_t b = f->off[d], e = f->off[d + 1];
+        if (e <= b) continue;
+        hipLaunchKernelGGL((k_prolong<float, true>), dim3(blocks_for(e - b, kT)), dim3(kT), 0, st, b, e, f->pro_ptr.p,
+                           f->pro_idx.p, f->p.p, (const float*)f->z.p, f->g.p, (const double*)f->scal.p,
+                           (const int*)f->ctl.p);
+        ITS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_apply_down, dim3(red_blocks(n)), dim3(kT), 0, st, n, f->down.p, f->g.p, f->t.p, f->part.p,
+                       f->ticket.p, f->scal.p, f->ctl.p);
+    ITS_LAUNCH_CHECK();
+    launch_AT32(f, st, 1, 1);
+    hipLaunchKernelGGL(k_precond, dim3(red_blocks(n)), dim3(kT), 0, st, n, f->h.p, f->invd32.p, f->z.p, f->part.p,
+                       f->ticket.p, f->scal.p, f->ctl.p, 0);
+    ITS_LAUNCH_CHECK();
+}
+
+constexpr int kChunk = 4;
+
+// inner binary32 CGLS on rhs (device binary64); returns iterations used
+int64_t inner_cgls(Fit* f, const double* rhs, double rel_tol, int64_t max_it, hipStream_t st) {
+    const int64_t n = f->n;
+    int ctl_init[4] = {0, 0, 0, (int)std::min<int64_t>(max_it, 1 << 30)};
+    ITS_HIP(hipMemcpyAsync(f->ctl.p, ctl_init, sizeof ctl_init, hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemsetAsync(f->x.p, 0, n * sizeof(float), st));
+    ITS_HIP(hipMemsetAsync(f->p.p, 0, n * sizeof(float), st));
+    hipLaunchKernelGGL(k_to32, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, rhs, f->r.p);
+    ITS_LAUNCH_CHECK();
+    launch_AT32(f, st, 0, 0);
+    // z = M nr, absNew, rn2_0 (threshold not yet known: set +inf-free 0 first)
+    ITS_HIP(hipMemsetAsync(f->scal.p + S_THRESH, 0, sizeof(double), st));
+    hipLaunchKernelGGL(k_precond, dim3(red_blocks(n)), dim3(kT), 0, st, n, f->h.p, f->invd32.p, f->z.p, f->part.p,
+                       f->ticket.p, f->scal.p, f->ctl.p, 1);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_set_thresh, dim3(1), dim3(1), 0, st, f->scal.p, rel_tol * rel_tol);
+    ITS_LAUNCH_CHECK();
+    // graph of kChunk iterations, captured once per octree
+    if (!f->gexec) {
+        ITS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < kChunk; ++k) launch_iteration(f, st);
+        ITS_HIP(hipStreamEndCapture(st, &f->graph));
+        ITS_HIP(hipGraphInstantiate(&f->gexec, f->graph, nullptr, nullptr, 0));
+    }
+    int ctl[2] = {0, 0};
+    int64_t launched = 0;
+    while (launched < max_it + kChunk) {
+        ITS_HIP(hipGraphLaunch(f->gexec, st));
+        launched += kChunk;
+        ITS_HIP(hipMemcpyAsync(ctl, f->ctl.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+        if (ctl[0]) break;
+    }
+    return (int64_t)ctl[1];
+}
+}  // namespace
+
+void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose) {
+    Fit* f = fit_get(s);
+    hipStream_t st = s->ctx->stream;
+    if (transpose) apply_AT64(f, d_x, d_y, st);
+    else apply_A64(f, d_x, d_y, st);
+    ITS_HIP(hipStreamSynchronize(st));
+}
+
+void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int64_t max_iter, bool warm,
+                its_lscg_stats* out) {
+    auto t0 = std::chrono::steady_clock::now();
+    Fit* f = fit_get(s);
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = f->n;
+    if (max_iter <= 0) max_iter = 2 * n;
+    if (!warm) ITS_HIP(hipMemsetAsync(d_lambda, 0, n * sizeof(double), st));
+    // ||A^T b||^2
+    apply_AT64(f, d_b, f->h64.p, st);
+    double rhs2 = dot64(f, f->h64.p, f->h64.p, st);
+    double bb = dot64(f, d_b, d_b, st);
+    its_lscg_stats S{};
+    int64_t iters = 0, outer = 0;
+    double rel = 0, rel0 = -1;
+    if (rhs2 == 0) {
+        ITS_HIP(hipMemsetAsync(d_lambda, 0, n * sizeof(double), st));
+    } else {
+        const double inner_tol = 1e-3;
+        for (;;) {
+            apply_A64(f, d_lambda, f->tmp64.p, st);
+            hipLaunchKernelGGL(k_sub64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, d_b, f->tmp64.p, f->r64.p);
+            ITS_LAUNCH_CHECK();
+            apply_AT64(f, f->r64.p, f->nr64.p, st);
+            double rn2 = dot64(f, f->nr64.p, f->nr64.p, st);
+            rel = std::sqrt(rn2 / rhs2);
+            if (rel0 < 0) rel0 = rel;
+            if (rel < tol || iters >= max_iter) break;
+            double want = std::max(inner_tol, 0.5 * tol / rel);
+            if (want >= 1.0) want = 0.5;
+            int64_t it = inner_cgls(f, f->r64.p, want, max_iter - iters, st);
+            iters += it;
+            hipLaunchKernelGGL(k_axpy_f32_to64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->x.p, d_lambda);
+            ITS_LAUNCH_CHECK();
+            ++outer;
+            if (it == 0) break;   // no progress possible
+        }
+    }
+    // final residuals
+    apply_A64(f, d_lambda, f->tmp64.p, st);
+    hipLaunchKernelGGL(k_sub64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, d_b, f->tmp64.p, f->r64.p);
+    ITS_LAUNCH_CHECK();
+    double rr = dot64(f, f->r64.p, f->r64.p, st);
+    ITS_HIP(hipStreamSynchronize(st));
+    auto t1 = std::chrono::steady_clock::now();
+    S.iterations = iters;
+    S.refinements = outer > 0 ? outer - 1 : 0;
+    S.normal_rel = rel;
+    S.initial_normal_rel = rel0 < 0 ? 0 : rel0;
+    S.residual_abs = std::sqrt(rr);
+    S.residual_rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    S.seconds = std::chrono::duration<double>(t1 - t0).count();
+    S.converged = rel < tol;
+    if (out) *out = S;
+}
+
+}  // namespace its

@@ -1,0 +1,503 @@
+// Sparse voxel octree construction for gfx950.
+//
+// Replaces SparseVoxelOctree::createOctree (reference src/cuAcc/SVO.cu:674-856),
+// constructNodeNeighbors (858-911) and the per-level vertex dedup
+// (913-963, 1010-1096).  The reference keeps a dense Morton-indexed uint32
+// occupancy array and runs thrust scans/sorts over it level by level.  Here
+// every set of cells or lattice points is a bitset with a rank directory
+// (RankWord), so compaction is a popcount scan, membership and rank are one
+// word read, and nothing is sorted: Morton order of the bitset is the order
+// of the reference's compacted arrays, and the x-major lattice bitset order
+// is the reference's lexicographic vertex order.
+//
+// Floating point: node origins and voxel tests are binary64 in exactly the
+// operation order of the reference expressions, compiled without FMA
+// contraction, so they agree bit for bit with the CPU restatement.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace its {
+namespace {
+
+constexpr int kT = 256;
+
+// ------------------------------------------------------------ voxelizer ----
+struct Grid {
+    double go[3];
+    double unit[3];
+    int R;
+};
+
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+__device__ inline double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]); }
+__device__ inline double dot2(double a0, double a1, double b0, double b1) { return a0 * b0 + a1 * b1; }
+
+// triangle's clamped voxel box (surfaceVoxelize, SVO.cu:394-404)
+__device__ inline void tri_box(const double* t, const Grid& g, int bo[3], int be[3]) {
+    for (int a = 0; a < 3; ++a) {
+        double v0 = t[a] - g.go[a], v1 = t[3 + a] - g.go[a], v2 = t[6 + a] - g.go[a];
+        double lo = dmin(v0, dmin(v1, v2)), hi = dmax(v0, dmax(v1, v2));
+        int l = (int)(lo / g.unit[a]), h = (int)(hi / g.unit[a]);
+        bo[a] = max(0, min(l, g.R - 1));
+        be[a] = max(0, min(h, g.R - 1));
+    }
+}
+
+__global__ void k_tri_count(const double* tris, int64_t T, Grid g, uint64_t* cnt) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int bo[3], be[3];
+    tri_box(tris + 9 * t, g, bo, be);
+    cnt[t] = (uint64_t)(be[0] - bo[0] + 1) * (uint64_t)(be[1] - bo[1] + 1) * (uint64_t)(be[2] - bo[2] + 1);
+}
+
+// one thread per (triangle, candidate voxel); the triangle set-up is
+// recomputed per candidate, in the reference's expression order
+__global__ void k_voxelize(const double* tris, int64_t T, const uint64_t* off, uint64_t total, Grid g,
+                           uint32_t* bits) {
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = T - 1;   // last t with off[t] <= c
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const double* tri = tris + 9 * lo;
+        int bo[3], be[3];
+        tri_box(tri, g, bo, be);
+        uint64_t local = c - off[lo];
+        uint64_t nx = be[0] - bo[0] + 1, ny = be[1] - bo[1] + 1;
+        int x = bo[0] + (int)(local % nx);
+        int y = bo[1] + (int)((local / nx) % ny);
+        int z = bo[2] + (int)(local / (nx * ny));
+
+        double v0[3], v1[3], v2[3], e0[3], e1[3], e2[3], n[3];
+        for (int a = 0; a < 3; ++a) {
+            v0[a] = tri[a] - g.go[a];
+            v1[a] = tri[3 + a] - g.go[a];
+            v2[a] = tri[6 + a] - g.go[a];
+        }
+        for (int a = 0; a < 3; ++a) {
+            e0[a] = v1[a] - v0[a];
+            e1[a] = v2[a] - v1[a];
+            e2[a] = v0[a] - v2[a];
+        }
+        n[0] = e0[1] * e1[2] - e0[2] * e1[1];
+        n[1] = e0[2] * e1[0] - e0[0] * e1[2];
+        n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+        double z2 = dot3(n, n);
+        if (z2 > 0.0) {
+            double s = sqrt(z2);
+            n[0] = n[0] / s;
+            n[1] = n[1] / s;
+            n[2] = n[2] / s;
+        }
+        const double* u = g.unit;
+        double cc[3] = {n[0] > 0.0 ? u[0] : 0.0, n[1] > 0.0 ? u[1] : 0.0, n[2] > 0.0 ? u[2] : 0.0};
+        double cv[3], dv[3];
+        for (int a = 0; a < 3; ++a) {
+            cv[a] = cc[a] - v0[a];
+            dv[a] = (u[a] - cc[a]) - v0[a];
+        }
+        double d1 = dot3(n, cv), d2 = dot3(n, dv);
+        double p[3] = {(double)x * u[0], (double)y * u[1], (double)z * u[2]};
+        double nDOTp = dot3(n, p);
+        if ((nDOTp + d1) * (nDOTp + d2) > 0.0) continue;
+
+        const double* vs[3] = {v0, v1, v2};
+        const double* es[3] = {e0, e1, e2};
+        bool ok = true;
+        // XY
+        for (int i = 0; i < 3 && ok; ++i) {
+            double a0 = -1.0 * es[i][1], a1 = es[i][0];
+            if (n[2] < 0.0) { a0 = -a0; a1 = -a1; }
+            double d = (-1.0 * dot2(a0, a1, vs[i][0], vs[i][1])) + dmax(0.0, u[0] * a0) + dmax(0.0, u[1] * a1);
+            if ((dot2(a0, a1, p[0], p[1]) + d) < 0.0) ok = false;
+        }
+        // YZ
+        for (int i = 0; i < 3 && ok; ++i) {
+            double a0 = -1.0 * es[i][2], a1 = es[i][1];
+            if (n[0] < 0.0) { a0 = -a0; a1 = -a1; }
+            double d = (-1.0 * dot2(a0, a1, vs[i][1], vs[i][2])) + dmax(0.0, u[1] * a0) + dmax(0.0, u[2] * a1);
+            if ((dot2(a0, a1, p[1], p[2]) + d) < 0.0) ok = false;
+        }
+        // ZX
+        for (int i = 0; i < 3 && ok; ++i) {
+            double a0 = -1.0 * es[i][0], a1 = es[i][2];
+            if (n[1] < 0.0) { a0 = -a0; a1 = -a1; }
+            double d = (-1.0 * dot2(a0, a1, vs[i][2], vs[i][0])) + dmax(0.0, u[2] * a0) + dmax(0.0, u[0] * a1);
+            if ((dot2(a0, a1, p[2], p[0]) + d) < 0.0) ok = false;
+        }
+        if (!ok) continue;
+        uint32_t m = morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
+        atomicOr(&bits[m >> 5], 1u << (m & 31));
+    }
+}
+
+// set bits of a RankWord bitset -> their indices, ascending
+__global__ void k_emit_bits(const RankWord* g, int64_t nwords, uint32_t* out) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    RankWord rw = g[w];
+    uint32_t b = rw.bits, r = rw.rank;
+    while (b) {
+        int i = __ffs(b) - 1;
+        out[r++] = (uint32_t)(w * 32 + i);
+        b &= b - 1;
+    }
+}
+
+// bits (uint32 words) -> RankWord array (rank filled later)
+__global__ void k_pack_bits(const uint32_t* bits, int64_t nwords, RankWord* g) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwords) g[w] = RankWord{bits[w], 0u};
+}
+
+// ------------------------------------------------------------ levels -------
+__global__ void k_heads(const uint32_t* occ, int64_t n, uint32_t* head) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || (occ[i] >> 3) != (occ[i - 1] >> 3)) ? 1u : 0u;
+}
+__global__ void k_parents(const uint32_t* occ, int64_t n, uint32_t* prank, uint32_t* parents) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (i == 0 || (occ[i] >> 3) != (occ[i - 1] >> 3)) ? 1u : 0u;
+    uint32_t r = prank[i] + h - 1u;   // exclusive count of heads + own head - 1
+    prank[i] = r;
+    if (h) parents[r] = (occ[i] >> 3) & 0x0FFFFFFFu;
+}
+
+struct NodeOut {
+    uint32_t* morton;
+    uint8_t* is_leaf;
+    double* origin;
+    double* width;
+    uint32_t* parent;
+    uint32_t* childbase;
+};
+
+// nodes of one level: octets of the children of each occupied parent
+// (createNode_1/2 + createRemainNode, SVO.cu:560-672)
+__global__ void k_nodes_fill(int64_t n_nodes, const uint32_t* parents, uint32_t single_code, int64_t off, double gx,
+                             double gy, double gz, double w, NodeOut o) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_nodes) return;
+    uint32_t m = (n_nodes == 1) ? single_code : ((parents[j >> 3] << 3) | (uint32_t)(j & 7));
+    int64_t g = off + j;
+    o.morton[g] = m;
+    o.is_leaf[g] = 1;
+    uint32_t x, y, z;
+    demorton3(m, x, y, z);
+    o.origin[3 * g + 0] = gx + w * (double)x;
+    o.origin[3 * g + 1] = gy + w * (double)y;
+    o.origin[3 * g + 2] = gz + w * (double)z;
+    o.width[g] = w;
+    o.parent[g] = ITS_NONE;
+    o.childbase[g] = ITS_NONE;
+}
+
+__global__ void k_nodes_occ(int64_t n_occ, const uint32_t* occ, const uint32_t* prank, int64_t n_nodes, int level,
+                            int64_t off, int64_t child_off, NodeOut o) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_occ) return;
+    int64_t addr = (n_nodes == 1) ? 0 : 8 * (int64_t)prank[i] + (occ[i] & 7);
+    int64_t g = off + addr;
+    if (level > 0) {
+        o.is_leaf[g] = 0;
+        uint32_t cb = (uint32_t)(child_off + 8 * i);
+        o.childbase[g] = cb;
+        for (int c = 0; c < 8; ++c) o.parent[cb + c] = (uint32_t)g;
+    }
+}
+
+// neighbour index k = (dx+1) + 3(dy+1) + 9(dz+1); child key x | y<<1 | z<<2.
+// Through the parent's neighbour's child, as findNeighbors (SVO.cu:858-890).
+__constant__ uint8_t c_lut_parent[8 * 27];
+__constant__ uint8_t c_lut_child[8 * 27];
+
+__global__ void k_neighbors(int64_t n, int64_t off, const uint32_t* morton, const uint32_t* parent,
+                            const uint32_t* childbase, uint32_t* nb) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t g = off + i;
+    uint32_t p = parent[g];
+    int key = morton[g] & 7;
+    for (int k = 0; k < 27; ++k) {
+        uint32_t pn = nb[27 * (int64_t)p + c_lut_parent[key * 27 + k]];
+        uint32_t v = ITS_NONE;
+        if (pn != ITS_NONE) {
+            uint32_t cb = childbase[pn];
+            v = (cb == ITS_NONE) ? ITS_NONE : cb + c_lut_child[key * 27 + k];
+        }
+        nb[27 * g + k] = v;
+    }
+}
+
+__global__ void k_root_self(uint32_t* nb, int64_t top) { nb[27 * top + 13] = (uint32_t)top; }
+
+// ------------------------------------------------------------ vertices -----
+__device__ inline uint32_t lat_index(uint32_t X, uint32_t Y, uint32_t Z, uint32_t D) { return (X * D + Y) * D + Z; }
+
+__global__ void k_mark_corners(int64_t n, const uint32_t* morton, int64_t off, uint32_t D, uint32_t* bits) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z;
+    demorton3(morton[off + i], x, y, z);
+    for (int c = 0; c < 8; ++c) {
+        uint32_t idx = lat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), D);
+        atomicOr(&bits[idx >> 5], 1u << (idx & 31));
+    }
+}
+
+__global__ void k_emit_verts(const RankWord* g, int64_t nwords, uint32_t D, int64_t voff, int level, uint16_t* vx,
+                             uint16_t* vy, uint16_t* vz, uint8_t* vlev, uint32_t* owner) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    RankWord rw = g[w];
+    uint32_t b = rw.bits, r = rw.rank;
+    while (b) {
+        int bi = __ffs(b) - 1;
+        uint32_t idx = (uint32_t)(w * 32 + bi);
+        int64_t v = voff + r++;
+        vx[v] = (uint16_t)(idx / (D * D));
+        vy[v] = (uint16_t)((idx / D) % D);
+        vz[v] = (uint16_t)(idx % D);
+        vlev[v] = (uint8_t)level;
+        owner[v] = ITS_NONE;
+        b &= b - 1;
+    }
+}
+
+// owner = smallest global index of the nodes sharing the corner (the min over
+// d_vertSharedLUT neighbours in determineNodeVertex, SVO.cu:950-960)
+__global__ void k_owner(int64_t n, const uint32_t* morton, int64_t off, uint32_t D, const RankWord* lat, int64_t voff,
+                        uint32_t* owner, uint32_t* node_corner) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t g = off + i;
+    uint32_t x, y, z;
+    demorton3(morton[g], x, y, z);
+    for (int c = 0; c < 8; ++c) {
+        uint32_t idx = lat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), D);
+        uint32_t v = (uint32_t)(voff + bit_rank(lat, idx));
+        atomicMin(&owner[v], (uint32_t)g);
+        node_corner[8 * g + c] = v;
+    }
+}
+
+// corner position from the owner node: origin + width * (offset), the copy
+// the reference's stable sort + unique keeps (SVO.cu:1081-1082)
+__global__ void k_vxyz(int64_t nv, int64_t voff, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz,
+                       const uint32_t* owner, const uint32_t* morton, const double* origin, const double* width,
+                       double* xyz) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    int64_t v = voff + i;
+    uint32_t o = owner[v];
+    uint32_t x, y, z;
+    demorton3(morton[o], x, y, z);
+    double w = width[o];
+    xyz[3 * v + 0] = origin[3 * (int64_t)o + 0] + w * (double)(vx[v] - x);
+    xyz[3 * v + 1] = origin[3 * (int64_t)o + 1] + w * (double)(vy[v] - y);
+    xyz[3 * v + 2] = origin[3 * (int64_t)o + 2] + w * (double)(vz[v] - z);
+}
+
+void upload_luts() {
+    static bool done = false;
+    static int dev_done = -1;
+    int dev;
+    ITS_HIP(hipGetDevice(&dev));
+    if (done && dev_done == dev) return;
+    uint8_t lp[8 * 27], lc[8 * 27];
+    for (int key = 0; key < 8; ++key) {
+        int c[3] = {key & 1, (key >> 1) & 1, (key >> 2) & 1};
+        for (int k = 0; k < 27; ++k) {
+            int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+            int pk = 0, ck = 0, mul = 1;
+            for (int a = 0; a < 3; ++a) {
+                int t = c[a] + d[a];
+                int pd = (t < 0) ? -1 : (t > 1 ? 1 : 0);
+                pk += (pd + 1) * mul;
+                ck |= (t & 1) << a;
+                mul *= 3;
+            }
+            lp[key * 27 + k] = (uint8_t)pk;
+            lc[key * 27 + k] = (uint8_t)ck;
+        }
+    }
+    ITS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lut_parent), lp, sizeof lp));
+    ITS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lut_child), lc, sizeof lc));
+    done = true;
+    dev_done = dev;
+}
+
+}  // namespace
+
+void svo_build(its_svo s, const double* d_tris, int64_t T) {
+    hipStream_t st = s->ctx->stream;
+    const int R = s->R;
+    upload_luts();
+    Grid g;
+    for (int a = 0; a < 3; ++a) {
+        g.go[a] = s->origin[a];
+        g.unit[a] = s->width[a] / R;   // unitVoxelSize (SVO.cu:683-685)
+    }
+    g.R = R;
+    s->voxel_width = g.unit[0];        // unitNodeWidth = unitVoxelSize.x()
+
+    // ---- voxelize into a Morton-ordered bitset
+    const int64_t ncell = (int64_t)R * R * R;
+    const int64_t nwords0 = std::max<int64_t>(1, (ncell + 31) / 32);
+    DBuf<uint32_t> bits(nwords0);
+    ITS_HIP(hipMemsetAsync(bits.p, 0, bits.bytes(), st));
+    if (T > 0) {
+        DBuf<uint64_t> cnt(T);
+        hipLaunchKernelGGL(k_tri_count, dim3(blocks_for(T, kT)), dim3(kT), 0, st, d_tris, T, g, cnt.p);
+        ITS_LAUNCH_CHECK();
+        uint64_t total = scan_exclusive_u64(cnt.p, cnt.p, T, st);
+        if (total) {
+            unsigned nb = (unsigned)std::min<uint64_t>((total + kT - 1) / kT, 1u << 20);
+            hipLaunchKernelGGL(k_voxelize, dim3(nb), dim3(kT), 0, st, d_tris, T, cnt.p, total, g, bits.p);
+            ITS_LAUNCH_CHECK();
+        }
+    }
+    Level& l0 = s->lv[0];
+    {
+        DBuf<RankWord> occbits(nwords0);
+        hipLaunchKernelGGL(k_pack_bits, dim3(blocks_for(nwords0, kT)), dim3(kT), 0, st, bits.p, nwords0, occbits.p);
+        ITS_LAUNCH_CHECK();
+        int64_t nvox = (int64_t)rank_build(occbits.p, nwords0, st);
+        s->num_voxels = nvox;
+        l0.occ.alloc(std::max<int64_t>(nvox, 1));
+        hipLaunchKernelGGL(k_emit_bits, dim3(blocks_for(nwords0, kT)), dim3(kT), 0, st, occbits.p, nwords0, l0.occ.p);
+        ITS_LAUNCH_CHECK();
+        l0.n_occ = nvox;
+    }
+    bits.release();
+
+    // ---- level loop (SVO.cu:717-815): a level with a single occupied cell
+    // is the root
+    int depth = 0;
+    double w = g.unit[0];
+    while (s->lv[depth].n_occ > 0) {
+        Level& L = s->lv[depth];
+        L.R = std::max(1, R >> depth);
+        L.width = w;
+        ++depth;
+        if (L.n_occ == 1) {
+            L.n_nodes = 1;
+            break;
+        }
+        ITS_REQUIRE(depth < kMaxLevels, ITS_E_RANGE, "octree deeper than supported");
+        L.prank.alloc(L.n_occ);
+        hipLaunchKernelGGL(k_heads, dim3(blocks_for(L.n_occ, kT)), dim3(kT), 0, st, L.occ.p, L.n_occ, L.prank.p);
+        ITS_LAUNCH_CHECK();
+        int64_t np = (int64_t)scan_exclusive_u32(L.prank.p, L.prank.p, L.n_occ, st);
+        Level& P = s->lv[depth];
+        P.occ.alloc(np);
+        P.n_occ = np;
+        hipLaunchKernelGGL(k_parents, dim3(blocks_for(L.n_occ, kT)), dim3(kT), 0, st, L.occ.p, L.n_occ, L.prank.p,
+                           P.occ.p);
+        ITS_LAUNCH_CHECK();
+        L.n_nodes = 8 * np;
+        w *= 2.0;
+    }
+    s->depth = depth;
+    int64_t total = 0;
+    for (int d = 0; d < depth; ++d) {
+        s->esum_nodes[d] = total;
+        total += s->lv[d].n_nodes;
+    }
+    s->esum_nodes[depth] = total;
+    s->num_nodes = total;
+    if (depth == 0) return;
+
+    // ---- node arrays
+    s->morton.alloc(total);
+    s->is_leaf.alloc(total);
+    s->node_origin.alloc(3 * total);
+    s->node_width.alloc(total);
+    s->parent.alloc(total);
+    s->childbase.alloc(total);
+    NodeOut o{s->morton.p, s->is_leaf.p, s->node_origin.p, s->node_width.p, s->parent.p, s->childbase.p};
+    for (int d = 0; d < depth; ++d) {
+        Level& L = s->lv[d];
+        const uint32_t* parents = (d + 1 < depth) ? s->lv[d + 1].occ.p : nullptr;
+        uint32_t single = (L.n_nodes == 1) ? read1(L.occ.p, st) : 0u;
+        hipLaunchKernelGGL(k_nodes_fill, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, parents, single,
+                           s->esum_nodes[d], g.go[0], g.go[1], g.go[2], L.width, o);
+        ITS_LAUNCH_CHECK();
+    }
+    for (int d = 0; d < depth; ++d) {
+        Level& L = s->lv[d];
+        hipLaunchKernelGGL(k_nodes_occ, dim3(blocks_for(L.n_occ, kT)), dim3(kT), 0, st, L.n_occ, L.occ.p,
+                           L.n_nodes == 1 ? nullptr : L.prank.p, L.n_nodes, d, s->esum_nodes[d],
+                           d > 0 ? s->esum_nodes[d - 1] : 0, o);
+        ITS_LAUNCH_CHECK();
+    }
+
+    // ---- neighbours, top-down
+    s->neighbors.alloc(27 * total);
+    ITS_HIP(hipMemsetAsync(s->neighbors.p, 0xFF, s->neighbors.bytes(), st));
+    hipLaunchKernelGGL(k_root_self, dim3(1), dim3(1), 0, st, s->neighbors.p, total - 1);
+    ITS_LAUNCH_CHECK();
+    for (int d = depth - 2; d >= 0; --d) {
+        Level& L = s->lv[d];
+        hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, s->esum_nodes[d],
+                           s->morton.p, s->parent.p, s->childbase.p, s->neighbors.p);
+        ITS_LAUNCH_CHECK();
+    }
+
+    // ---- vertices per level
+    int64_t nv_total = 0;
+    std::vector<int64_t> lat_words(depth);
+    for (int d = 0; d < depth; ++d) {
+        Level& L = s->lv[d];
+        const uint32_t D = (uint32_t)L.R + 1;
+        const int64_t nbits = (int64_t)D * D * D;
+        const int64_t nwords = (nbits + 31) / 32;
+        lat_words[d] = nwords;
+        DBuf<uint32_t> lb(nwords);
+        ITS_HIP(hipMemsetAsync(lb.p, 0, lb.bytes(), st));
+        hipLaunchKernelGGL(k_mark_corners, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, s->morton.p,
+                           s->esum_nodes[d], D, lb.p);
+        ITS_LAUNCH_CHECK();
+        L.lattice.alloc(nwords);
+        hipLaunchKernelGGL(k_pack_bits, dim3(blocks_for(nwords, kT)), dim3(kT), 0, st, lb.p, nwords, L.lattice.p);
+        ITS_LAUNCH_CHECK();
+        L.n_verts = (int64_t)rank_build(L.lattice.p, nwords, st);
+        s->esum_verts[d] = nv_total;
+        nv_total += L.n_verts;
+    }
+    s->esum_verts[depth] = nv_total;
+    s->num_verts = nv_total;
+    s->vx.alloc(nv_total);
+    s->vy.alloc(nv_total);
+    s->vz.alloc(nv_total);
+    s->vlevel.alloc(nv_total);
+    s->vowner.alloc(nv_total);
+    s->vxyz.alloc(3 * nv_total);
+    s->node_corner.alloc(8 * total);
+    for (int d = 0; d < depth; ++d) {
+        Level& L = s->lv[d];
+        const uint32_t D = (uint32_t)L.R + 1;
+        hipLaunchKernelGGL(k_emit_verts, dim3(blocks_for(lat_words[d], kT)), dim3(kT), 0, st, L.lattice.p, lat_words[d],
+                           D, s->esum_verts[d], d, s->vx.p, s->vy.p, s->vz.p, s->vlevel.p, s->vowner.p);
+        ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_owner, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, s->morton.p,
+                           s->esum_nodes[d], D, L.lattice.p, s->esum_verts[d], s->vowner.p, s->node_corner.p);
+        ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_vxyz, dim3(blocks_for(L.n_verts, kT)), dim3(kT), 0, st, L.n_verts, s->esum_verts[d],
+                           s->vx.p, s->vy.p, s->vz.p, s->vowner.p, s->morton.p, s->node_origin.p, s->node_width.p,
+                           s->vxyz.p);
+        ITS_LAUNCH_CHECK();
+    }
+    ITS_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace its

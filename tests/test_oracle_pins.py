@@ -1,0 +1,373 @@
+"""Pins for the CPU oracle (oracle/its_oracle.c).
+
+The reference ships no tests and no golden vectors (its only data file is
+model/bunny.off).  Each oracle function is pinned by one of the following:
+- the reference's own tables, parsed as data from its sources when
+  /root/reference is mounted;
+- closed forms;
+- brute force on small inputs.
+Every check is chosen so that a plausible slip in the oracle makes it fail:
+a swapped axis, an off-by-one level, a missing term, or a wrong sign.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import REFERENCE_SRC, case, fitted
+
+needs_ref = pytest.mark.skipif(not os.path.isdir(REFERENCE_SRC), reason="reference sources not mounted")
+
+
+def _c_array(text, name):
+    """Integers of the C array initialiser `name[...] = { ... };`."""
+    m = re.search(re.escape(name) + r"\s*(\[[^\]]*\])+\s*=\s*\{", text)
+    assert m, name
+    depth, i = 1, m.end()
+    while depth:
+        depth += {"{": 1, "}": -1}.get(text[i], 0)
+        i += 1
+    body = text[m.end():i - 1]
+    return [int(x, 0) for x in re.findall(r"-?0x[0-9A-Fa-f]+|-?\d+", body)]
+
+
+def _read(rel):
+    with open(os.path.join(REFERENCE_SRC, rel), encoding="latin-1") as fh:
+        return fh.read()
+
+
+# ------------------------------------------------------------------ Morton
+@needs_ref
+def test_morton_tables_equal_reference_text():
+    ex, ey, ez, dx, dy, dz = O.morton_tables()
+    src = _read("MortonLUT.h")
+    assert _c_array(src, "h_morton256_x") == list(ex)
+    assert _c_array(src, "h_morton256_y") == list(ey)
+    assert _c_array(src, "h_morton256_z") == list(ez)
+    assert _c_array(src, "h_Morton3D_decode_x_512") == list(dx)
+    assert _c_array(src, "h_Morton3D_decode_y_512") == list(dy)
+    assert _c_array(src, "h_Morton3D_decode_z_512") == list(dz)
+    svo = _read("cuAcc/SVO.cu")
+    assert _c_array(svo, "d_morton256_x") == list(ex)
+
+
+def test_morton_closed_form():
+    rng = np.random.default_rng(0)
+    for x, y, z in rng.integers(0, 1024, size=(2000, 3)):
+        m = 0
+        for b in range(10):
+            m |= ((int(x) >> b) & 1) << (3 * b) | ((int(y) >> b) & 1) << (3 * b + 1) | ((int(z) >> b) & 1) << (3 * b + 2)
+        assert O.morton_encode(int(x), int(y), int(z)) == m
+
+
+@needs_ref
+def test_neighbor_luts_equal_reference_text():
+    lp, lc, vs, es = O.neighbor_luts()
+    svo = _read("cuAcc/SVO.cu")
+    assert _c_array(svo, "neighbor_LUTparent") == list(lp.ravel())
+    assert _c_array(svo, "neighbor_LUTchild") == list(lc.ravel())
+    assert _c_array(svo, "d_vertSharedLUT") == list(vs)
+    assert _c_array(svo, "d_edgeSharedLUT") == list(es)
+
+
+def test_neighbor_luts_closed_form():
+    """child key c + offset d lands in parent neighbour floor((c+d)/2) at key (c+d) mod 2."""
+    lp, lc, vs, _ = O.neighbor_luts()
+    for key in range(8):
+        c = np.array([key & 1, (key >> 1) & 1, key >> 2])
+        for k in range(27):
+            d = np.array([k % 3 - 1, (k // 3) % 3 - 1, k // 9 - 1])
+            t = c + d
+            pd = np.floor_divide(t, 2)
+            assert lp[key, k] == (pd[0] + 1) + 3 * (pd[1] + 1) + 9 * (pd[2] + 1)
+            assert lc[key, k] == (t[0] % 2) | (t[1] % 2) << 1 | (t[2] % 2) << 2
+    # the 8 nodes sharing corner i all contain that corner point
+    for i in range(8):
+        ci = np.array([i & 1, (i >> 1) & 1, i >> 2])
+        for k in vs[8 * i:8 * i + 8]:
+            d = np.array([k % 3 - 1, (k // 3) % 3 - 1, k // 9 - 1])
+            rel = ci - d
+            assert np.all((rel >= 0) & (rel <= 1))
+
+
+# ------------------------------------------------------------------ MC tables
+def _product_tri_table():
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "its_amd", "csrc",
+                        "mc_tables.inc")
+    with open(path) as fh:
+        vals = _c_array(fh.read(), "kMcTri")
+    return np.array(vals).reshape(256, 16)
+
+
+@needs_ref
+def test_mc_tables_equal_reference_text():
+    src = _read("cuAcc/MarchingCubes/LookTable.h")
+    ref_tri = np.array(_c_array(src, "triTable")).reshape(256, 16)
+    ref_nv = np.array(_c_array(src, "nVertsTable"))
+    np.testing.assert_array_equal(O.tri_table(), ref_tri)
+    np.testing.assert_array_equal(O.nverts_table(), ref_nv)
+    np.testing.assert_array_equal(_product_tri_table(), ref_tri)
+
+
+def test_mc_table_consistency():
+    """Each case's triangles use exactly the edges whose corners straddle the iso value."""
+    tri = O.tri_table()
+    nv = O.nverts_table()
+    edges = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+    for c in range(256):
+        row = [e for e in tri[c] if e >= 0]
+        assert nv[c] == len(row) and len(row) % 3 == 0
+        cut = {e for e, (a, b) in enumerate(edges) if ((c >> a) & 1) != ((c >> b) & 1)}
+        assert set(row) == cut
+    np.testing.assert_array_equal(_product_tri_table(), tri)
+
+
+# ------------------------------------------------------------------ basis
+def test_hat_basis_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(5000):
+        c, w = rng.uniform(-1, 1), rng.uniform(0.01, 1)
+        x = c + rng.uniform(-1.5, 1.5) * w
+        v = O.lib().ora_base_function(x, c, w)
+        if abs(x - (c - w)) < 1e-9 or abs(x - (c + w)) < 1e-9:
+            assert v == 0.0
+        else:
+            assert abs(v - max(0.0, 1 - abs(x - c) / w)) < 1e-14
+
+
+# ------------------------------------------------------------------ voxelizer
+def _sat_overlap(tri, lo, hi):
+    """Brute-force separating-axis triangle / box test (Akenine-Moller)."""
+    c = (lo + hi) / 2
+    h = (hi - lo) / 2
+    v = tri - c
+    e = [v[1] - v[0], v[2] - v[1], v[0] - v[2]]
+    axes = [np.eye(3)[i] for i in range(3)] + [np.cross(e[0], e[1])]
+    axes += [np.cross(np.eye(3)[i], ej) for i in range(3) for ej in e]
+    for a in axes:
+        if np.dot(a, a) < 1e-30:
+            continue
+        p = v @ a
+        r = np.abs(a) @ h
+        if p.min() > r + 1e-12 or p.max() < -r - 1e-12:
+            return False
+    return True
+
+
+def test_voxelizer_against_separating_axis():
+    c = case("random", 16)
+    keys = set(O.voxelize(c.tris, c.origin, c.width, 16).tolist())
+    unit = c.width / 16
+    brute = set()
+    for t in c.tris[:120]:
+        T = t.reshape(3, 3) - c.origin
+        lo = np.clip((T.min(0) / unit).astype(int), 0, 15)
+        hi = np.clip((T.max(0) / unit).astype(int), 0, 15)
+        for x in range(lo[0], hi[0] + 1):
+            for y in range(lo[1], hi[1] + 1):
+                for z in range(lo[2], hi[2] + 1):
+                    if _sat_overlap(T, np.array([x, y, z]) * unit, (np.array([x, y, z]) + 1) * unit):
+                        brute.add(O.morton_encode(x, y, z))
+    sub = set(O.voxelize(c.tris[:120], c.origin, c.width, 16).tolist())
+    assert sub <= keys
+    # agreement except exact-boundary ties
+    assert len(sub ^ brute) <= max(2, len(brute) // 200)
+
+
+# ------------------------------------------------------------------ octree
+def _decode(m):
+    x = y = z = 0
+    for b in range(10):
+        x |= ((int(m) >> (3 * b)) & 1) << b
+        y |= ((int(m) >> (3 * b + 1)) & 1) << b
+        z |= ((int(m) >> (3 * b + 2)) & 1) << b
+    return x, y, z
+
+
+@pytest.mark.parametrize("name,R", [("icosphere", 16), ("random", 16), ("corner", 16), ("single", 8)])
+def test_octree_against_bruteforce(name, R):
+    """Levels, neighbours and vertices rebuilt from the voxel set alone."""
+    c = case(name, R)
+    s = O.SVO(c.tris, c.origin, c.width, R)
+    occ = [set(O.voxelize(c.tris, c.origin, c.width, R).tolist())]
+    while len(occ[-1]) > 1:
+        occ.append({m >> 3 for m in occ[-1]})
+    assert s.depth == len(occ)
+    off = np.concatenate([[0], np.cumsum(s.depth_num_nodes[: s.depth])])
+    w0 = c.width[0] / R
+    for d in range(s.depth):
+        m = s.morton[off[d]:off[d + 1]]
+        if d + 1 < s.depth:
+            expect = sorted(8 * p + k for p in occ[d + 1] for k in range(8))
+        else:
+            expect = sorted(occ[d])
+        assert list(m) == expect
+        # leaf flags: occupied cells above level 0 have children
+        leaf = s.is_leaf[off[d]:off[d + 1]]
+        assert all(bool(leaf[i]) == (d == 0 or int(mm) not in occ[d]) for i, mm in enumerate(m))
+        # origins: gridOrigin + width * xyz (binary64)
+        wd = w0 * 2 ** d
+        for i, mm in enumerate(m[:200]):
+            xyz = np.array(_decode(mm), float)
+            np.testing.assert_array_equal(s.node_origin[off[d] + i], c.origin + wd * xyz)
+        # neighbours: same-level node at offset d, if present
+        index = {int(mm): off[d] + i for i, mm in enumerate(m)}
+        for i, mm in enumerate(m[:300]):
+            x, y, z = _decode(mm)
+            for k in range(27):
+                dx, dy, dz = k % 3 - 1, (k // 3) % 3 - 1, k // 9 - 1
+                X, Y, Z = x + dx, y + dy, z + dz
+                j = 0xFFFFFFFF
+                if min(X, Y, Z) >= 0:
+                    j = index.get(O.morton_encode(X, Y, Z), 0xFFFFFFFF)
+                assert s.neighbors[off[d] + i, k] == j
+        # vertices: distinct node corners, lexicographic; owner = min node index
+        corners = {}
+        for i, mm in enumerate(m):
+            x, y, z = _decode(mm)
+            for k in range(8):
+                key = (x + (k & 1), y + ((k >> 1) & 1), z + (k >> 2))
+                corners[key] = min(corners.get(key, 1 << 40), off[d] + i)
+        keys = sorted(corners)
+        vb, ve = s.esum_verts[d], s.esum_verts[d + 1]
+        assert ve - vb == len(keys)
+        np.testing.assert_array_equal(s.vert_owner[vb:ve], [corners[k] for k in keys])
+        lat = np.rint((s.vert_xyz[vb:ve] - c.origin) / wd).astype(int)
+        np.testing.assert_array_equal(lat, np.array(keys))
+
+
+# ------------------------------------------------------------------ assembly
+def _dense_A(s):
+    """A_ij = B_j(v_i) over all pairs with the owner width (CUDACompute.cu:468)."""
+    n = s.num_verts
+    P = s.vert_xyz
+    W = s.node_width[s.vert_owner]
+    A = np.ones((n, n))
+    for a in range(3):
+        x = P[:, a][:, None]
+        cc = P[:, a][None, :]
+        w = W[None, :]
+        h = np.where(x <= cc, 1 + (x - cc) / w, 1 - (x - cc) / w)
+        dead = (np.abs(x - (cc - w)) < 1e-9) | (x < cc - w) | (np.abs(x - (cc + w)) < 1e-9) | (x > cc + w)
+        A *= np.where(dead, 0.0, h)
+    return A
+
+
+@pytest.mark.parametrize("name,R", [("single", 8), ("corner", 16), ("icosphere2", 8)])
+def test_assembly_equals_dense_collocation(name, R):
+    c = case(name, R)
+    s = O.SVO(c.tris, c.origin, c.width, R)
+    A = O.Matrix(s)
+    rp, cols, vals = A.csr()
+    M = np.zeros((A.n, A.n))
+    for i in range(A.n):
+        M[i, cols[rp[i]:rp[i + 1]]] = vals[rp[i]:rp[i + 1]]
+    D = _dense_A(s)
+    np.testing.assert_allclose(M, D, atol=1e-12, rtol=0)
+    rng = np.random.default_rng(3)
+    p, r = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    assert abs(A.mul(p) @ r - p @ A.tmul(r)) < 1e-10 * np.linalg.norm(p) * np.linalg.norm(r) * A.n
+
+
+# ------------------------------------------------------------------ LSCG
+@pytest.mark.parametrize("name,R", [("single", 8), ("corner", 16)])
+def test_lscg_converges_to_min_norm_least_squares(name, R):
+    F = fitted(name, R)
+    rp, cols, vals = F.A.csr()
+    M = np.zeros((F.A.n, F.A.n))
+    for i in range(F.A.n):
+        M[i, cols[rp[i]:rp[i + 1]]] = vals[rp[i]:rp[i + 1]]
+    # Preconditioned CGLS from x = 0 keeps its iterates in range(D A^T), with
+    # D = diag(1 / ||A_:j||^2) (Eigen's LeastSquareDiagonalPreconditioner).
+    # A is rank deficient (coincident vertices give equal rows), so the limit
+    # is the least-squares solution of minimum D^-1 norm:
+    # x = D^1/2 (A D^1/2)^+ b.
+    d = 1.0 / np.maximum((M * M).sum(0), 1e-300)
+    y = np.linalg.lstsq(M * np.sqrt(d)[None, :], F.b, rcond=None)[0]
+    x_w = np.sqrt(d) * y
+    np.testing.assert_allclose(F.lam_tight, x_w, atol=1e-8)
+    # and it is not the Euclidean minimum-norm solution (the rank deficiency is real)
+    x_ls = np.linalg.lstsq(M, F.b, rcond=None)[0]
+    assert np.abs(x_ls - x_w).max() > 1e-3
+    assert np.linalg.matrix_rank(M) < F.A.n
+    assert F.st["normal_rel"] < 1e-6
+
+
+def test_fit_interpolates_vertices():
+    """The commented-out check of ThinShells.cpp:629-652: f(v_i) = b_i."""
+    F = fitted("icosphere", 32)
+    f = F.svo.eval_dense(F.lam_tight, F.svo.vert_xyz[::7])
+    np.testing.assert_allclose(f, F.b[::7], atol=1e-9)
+
+
+# ------------------------------------------------------------------ evaluation
+@pytest.mark.parametrize("name,R", [("icosphere", 16), ("random", 16), ("corner", 16)])
+def test_support_eval_equals_dense(name, R):
+    F = fitted(name, R)
+    c = F.case
+    rng = np.random.default_rng(4)
+    q = rng.uniform(c.origin, c.end, size=(1500, 3))
+    fs = F.svo.eval_support(F.lam, q, c.end, 0.0)
+    fd = F.svo.eval_dense(F.lam, q)
+    np.testing.assert_allclose(fs, fd, atol=1e-12)
+    fq = F.svo.eval_dense_fast(F.lam, q)
+    np.testing.assert_allclose(fq, fd, atol=1e-12)
+    out = rng.uniform(c.origin - 0.5, c.end + 0.5, size=(500, 3))
+    np.testing.assert_allclose(F.svo.eval_dense_fast(F.lam, out), F.svo.eval_dense(F.lam, out), atol=1e-12)
+    fo = F.svo.eval_support(F.lam, out, c.end, 7.0)
+    outside = np.any((out <= c.origin) | (out >= c.end), axis=1)
+    assert np.all(fo[outside] == 7.0)
+
+
+def test_latent_eval_equals_support_on_mesh():
+    F = fitted("icosphere", 16)
+    c = F.case
+    fl, rc = F.svo.eval_latent(F.lam, c.V)
+    assert rc == 0
+    np.testing.assert_allclose(fl, F.svo.eval_support(F.lam, c.V, c.end, 0.0), atol=1e-12)
+
+
+# ------------------------------------------------------------------ marching cubes
+def test_mc_sphere_closed_and_on_lerp():
+    """f = |x| - r represented exactly... approximately by the fit of a
+    sphere SDF: the soup is closed and its vertices lie on lattice edges."""
+    F = fitted("icosphere", 16)
+    c = F.case
+    res = (24, 24, 24)
+    soup, cube, nvalid = O.marching_cubes(F.svo, F.lam, c.origin, c.width, res, 0.0)
+    assert len(soup) % 3 == 0 and len(soup) > 0
+    vs = c.width / 24
+    g = (soup - c.origin) / vs
+    on_lattice_axes = np.sum(np.abs(g - np.round(g)) < 1e-6, axis=1)
+    assert np.all(on_lattice_axes >= 2)   # each vertex lies on a lattice edge
+    key = np.round(soup / 1e-9).astype(np.int64)
+    _, inv = np.unique(key, axis=0, return_inverse=True)
+    T = inv.reshape(-1, 3)
+    E = np.sort(np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]]), axis=1)
+    E = E[E[:, 0] != E[:, 1]]
+    _, counts = np.unique(E, axis=0, return_counts=True)
+    assert np.all(counts == 2)
+
+
+# ------------------------------------------------------------------ signed distance
+def test_sdf_closed_forms():
+    from its_amd import meshes
+    V, Fc = meshes.icosphere(3)
+    rng = np.random.default_rng(6)
+    P = rng.uniform(-1.4, 1.4, size=(300, 3))
+    d = O.signed_distance(V, Fc, P)
+    r = np.linalg.norm(P, axis=1)
+    assert np.all(np.abs(d - (r - 1)) < 2e-2)
+    assert abs(O.winding_number(V, Fc, np.zeros(3)) - 1) < 1e-9
+    assert abs(O.winding_number(V, Fc, np.array([2.0, 0, 0]))) < 1e-9
+    # single triangle: brute force over a dense sampling of the triangle
+    T = np.array([[0.0, 0, 0], [1, 0.2, 0], [0.3, 1, 0.1]])
+    u, v = np.meshgrid(np.linspace(0, 1, 401), np.linspace(0, 1, 401))
+    m = (u + v) <= 1
+    S = T[0] + u[m][:, None] * (T[1] - T[0]) + v[m][:, None] * (T[2] - T[0])
+    for p in rng.uniform(-1, 2, size=(20, 3)):
+        dd = abs(O.signed_distance(T, np.array([[0, 1, 2]], np.int32), p[None])[0])
+        brute = np.sqrt(((S - p) ** 2).sum(1)).min()
+        assert dd <= brute + 1e-12 and brute - dd < 5e-3
