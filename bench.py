@@ -86,8 +86,12 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    stream = torch.cuda.current_stream(dev)
+    # one stream (default priority) for torch and the library, so the events
+    # bracket exactly the kernels of the step
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     ctx = _capi.Context(0)
+    ctx.set_stream(stream.cuda_stream)
 
     log("[bench] mesh")
     V, F, origin, end, width = make_mesh()

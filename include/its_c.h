@@ -169,6 +169,21 @@ int its_mc_extract(its_ctx ctx, its_svo svo, const double origin[3], const doubl
 int its_mc_lattice(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
                    float* values);
 
+/* ---- tunables (process-wide; set by the autotuner, its_amd/autotune.py) ----
+ * ITS_TUNE_DENSE_CELLS: the evaluation field collapses every level whose
+ *   cell count is at most this into one dense block grid (default 32768).
+ * ITS_TUNE_EVAL_BLOCKS_PER_CU: persistent evaluation grid, workgroups per CU
+ *   (default 8).
+ * ITS_TUNE_COARSE_VERTS: the LSCG cascade runs the top levels holding at most
+ *   this many vertices in one workgroup (default 2048, at most 12288).
+ * No reference counterpart (the reference hard-codes its launch shapes). */
+#define ITS_TUNE_DENSE_CELLS 0
+#define ITS_TUNE_EVAL_BLOCKS_PER_CU 1
+#define ITS_TUNE_COARSE_VERTS 2
+#define ITS_TUNE_COUNT 3
+int its_tune_set(int key, int64_t value);
+int its_tune_get(int key, int64_t* value);
+
 /* ---- timing helpers on the context stream (hipEvents) ---- */
 int its_timer_start(its_ctx ctx);
 int its_timer_stop(its_ctx ctx, double* ms);

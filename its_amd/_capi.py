@@ -18,6 +18,7 @@ ITS_OK = 0
 ITS_NONE = 0xFFFFFFFF
 ITS_MAX_LEVELS = 12
 EVAL_SUPPORT, EVAL_DENSE, EVAL_RAW = 0, 1, 2
+TUNE_DENSE_CELLS, TUNE_EVAL_BLOCKS_PER_CU, TUNE_COARSE_VERTS = 0, 1, 2
 
 
 class LibraryMissing(RuntimeError):
@@ -107,6 +108,8 @@ SIGNATURES = {
     "its_classify": [_f32p, C.c_int64, C.c_double, C.c_double, _i8p],
     "its_mc_extract": [vp, vp, _f64p, _f64p, _i32p, C.c_double, vp, C.c_int64, C.POINTER(McStats)],
     "its_mc_lattice": [vp, vp, _f64p, _f64p, _i32p, _f32p],
+    "its_tune_set": [C.c_int, C.c_int64],
+    "its_tune_get": [C.c_int, C.POINTER(C.c_int64)],
     "its_timer_start": [vp],
     "its_timer_stop": [vp, C.POINTER(C.c_double)],
 }
@@ -142,6 +145,16 @@ def check(rc):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def tune_set(key, value):
+    check(lib().its_tune_set(int(key), int(value)))
+
+
+def tune_get(key):
+    v = C.c_int64(0)
+    check(lib().its_tune_get(int(key), C.byref(v)))
+    return v.value
 
 
 def device_count():

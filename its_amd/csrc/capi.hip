@@ -12,6 +12,9 @@
 namespace its {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 8, 2048};
+int64_t tune_get(int key) { return g_tune[key]; }
 }  // namespace its
 
 using namespace its;
@@ -489,6 +492,23 @@ int its_mc_lattice(its_ctx ctx, its_svo s, const double origin[3], const double 
     ITS_REQUIRE(ctx == s->ctx && origin && width && res && values, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     mc_lattice(s, origin, width, res, values);
+    ITS_API_END
+}
+
+// ---- tunables
+int its_tune_set(int key, int64_t value) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(key >= 0 && key < ITS_TUNE_COUNT, ITS_E_INVALID, "unknown tunable");
+    ITS_REQUIRE(value >= 1, ITS_E_RANGE, "tunable must be positive");
+    if (key == ITS_TUNE_COARSE_VERTS) ITS_REQUIRE(value <= 12288, ITS_E_RANGE, "coarse vertex bound above 12288");
+    if (key == ITS_TUNE_EVAL_BLOCKS_PER_CU) ITS_REQUIRE(value <= 64, ITS_E_RANGE, "at most 64 workgroups per CU");
+    g_tune[key] = value;
+    ITS_API_END
+}
+int its_tune_get(int key, int64_t* value) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(key >= 0 && key < ITS_TUNE_COUNT && value, ITS_E_INVALID, "bad arguments");
+    *value = g_tune[key];
     ITS_API_END
 }
 

@@ -18,6 +18,17 @@
 // none of its descendants has one.  The walk therefore goes root -> leaf and
 // stops at the first unmarked cell.  Far-field queries touch only the small
 // coarse levels, which stay in L2.
+//
+// Level collapse: every level's hat sum is trilinear on each cell of any
+// finer level (two-scale relation).  So the whole contribution of levels
+// >= lc is the trilinear interpolation of one dense lattice grid G of its
+// values (the top-down prolongation cascade).  G is stored per level-lc cell
+// as one 32-byte block of its 8 corner values, so a query reads one aligned
+// cache line for all coarse levels; lc is the finest level whose cell count
+// stays under a tunable bound (32^3 cells, 1 MB, by default: L2 resident).
+// A query inside the box then walks only the lc sparse fine levels.  The
+// per-level walk over all levels remains for points outside the box (DENSE
+// and RAW modes).
 #include <algorithm>
 
 #include "internal.hpp"
@@ -64,13 +75,20 @@ __global__ void k_fill_blocks(int64_t nv, int64_t voff, const uint16_t* vx, cons
 struct EvalParams {
     const RankWord* cells[kMaxLevels];
     const float4* blocks[kMaxLevels];
+    const float4* G;  // per level-lc cell: 8 corner values of levels >= lc + child link (4 float4)
+    const float4* nodes[kMaxLevels];   // levels 1..lc-1: coefficients + child link (4 float4)
     int L;
     int R;
+    int lc;           // collapsed level
+    int Rc;           // cells per axis of G (R_lc)
     float ox, oy, oz, inv_w;
     double blo[3], bhi[3];   // accept box: open (SUPPORT) or closed (DENSE)
     int mode;
     float out_val;
 };
+
+// fine levels evaluated with all bitset words and blocks issued together
+constexpr int kFine = 6;
 
 struct SoaF32 {
     const float *x, *y, *z;
@@ -101,31 +119,30 @@ struct LatticeLoad {
 };
 
 __device__ inline float hat0(float t) { return fmaxf(0.f, 1.f - fabsf(t)); }
+__device__ inline float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-template <class Loader>
-__global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams P, float* out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    float x, y, z;
-    double dx, dy, dz;
-    load(i, x, y, z, dx, dy, dz);
-    if (P.mode == ITS_EVAL_SUPPORT) {
-        if (dx <= P.blo[0] || dy <= P.blo[1] || dz <= P.blo[2] || dx >= P.bhi[0] || dy >= P.bhi[1] || dz >= P.bhi[2]) {
-            out[i] = P.out_val;
-            return;
-        }
-    } else if (P.mode == ITS_EVAL_DENSE) {
-        if (dx < P.blo[0] || dy < P.blo[1] || dz < P.blo[2] || dx > P.bhi[0] || dy > P.bhi[1] || dz > P.bhi[2]) {
-            out[i] = P.out_val;
-            return;
-        }
-    }
-    const float ux = (x - P.ox) * P.inv_w, uy = (y - P.oy) * P.inv_w, uz = (z - P.oz) * P.inv_w;
-    const int Rm = P.R - 1;
-    const float fr = (float)Rm;   // clamp in float first: far or non-finite points stay defined
-    const int cx = (int)fminf(fmaxf(floorf(ux), 0.f), fr), cy = (int)fminf(fmaxf(floorf(uy), 0.f), fr),
-              cz = (int)fminf(fmaxf(floorf(uz), 0.f), fr);
-    const uint32_t m0 = morton3(cx, cy, cz);
+// trilinear interpolation inside a cell (t in [0,1]^3); b0 = corners z=0
+// (x | y<<1), b1 = corners z=1
+__device__ inline float tri_in(float4 b0, float4 b1, float tx, float ty, float tz) {
+    const float lo = lerpf(lerpf(b0.x, b0.y, tx), lerpf(b0.z, b0.w, tx), ty);
+    const float hi = lerpf(lerpf(b1.x, b1.y, tx), lerpf(b1.z, b1.w, tx), ty);
+    return lerpf(lo, hi, tz);
+}
+
+// same with clamped hats, for points outside the clamped cell
+__device__ inline float tri_clamped(float4 b0, float4 b1, float tx, float ty, float tz) {
+    const float wx0 = hat0(tx), wx1 = hat0(tx - 1.f);
+    const float wy0 = hat0(ty), wy1 = hat0(ty - 1.f);
+    const float wz0 = hat0(tz), wz1 = hat0(tz - 1.f);
+    const float lo = fmaf(wy1, fmaf(wx1, b0.w, wx0 * b0.z), wy0 * fmaf(wx1, b0.y, wx0 * b0.x));
+    const float hi = fmaf(wy1, fmaf(wx1, b1.w, wx0 * b1.z), wy0 * fmaf(wx1, b1.y, wx0 * b1.x));
+    return fmaf(wz1, hi, wz0 * lo);
+}
+
+__device__ inline float pow2neg(int d) { return __uint_as_float((uint32_t)(127 - d) << 23); }
+
+// every level, root -> leaf, clamped cells and hats: any point
+__device__ float eval_walk(const EvalParams& P, float ux, float uy, float uz, int cx, int cy, int cz, uint32_t m0) {
     float f = 0.f;
     for (int d = P.L - 1; d >= 0; --d) {
         const uint32_t m = m0 >> (3 * d);
@@ -133,17 +150,162 @@ __global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams 
         const uint32_t bit = 1u << (m & 31);
         if (!(w.bits & bit)) break;
         const uint32_t idx = w.rank + __popc(w.bits & (bit - 1u));
-        const float4 b0 = P.blocks[d][2 * idx], b1 = P.blocks[d][2 * idx + 1];
-        const float s = __uint_as_float((uint32_t)(127 - d) << 23);   // 2^-d
-        const float tx = ux * s - (float)(cx >> d), ty = uy * s - (float)(cy >> d), tz = uz * s - (float)(cz >> d);
-        const float wx0 = hat0(tx), wx1 = hat0(tx - 1.f);
-        const float wy0 = hat0(ty), wy1 = hat0(ty - 1.f);
-        const float wz0 = hat0(tz), wz1 = hat0(tz - 1.f);
-        const float lo = fmaf(wy1, fmaf(wx1, b0.w, wx0 * b0.z), wy0 * fmaf(wx1, b0.y, wx0 * b0.x));
-        const float hi = fmaf(wy1, fmaf(wx1, b1.w, wx0 * b1.z), wy0 * fmaf(wx1, b1.y, wx0 * b1.x));
-        f = fmaf(wz1, hi, fmaf(wz0, lo, f));
+        const float s = pow2neg(d);
+        f += tri_clamped(P.blocks[d][2 * idx], P.blocks[d][2 * idx + 1], ux * s - (float)(cx >> d),
+                         uy * s - (float)(cy >> d), uz * s - (float)(cz >> d));
     }
-    out[i] = f;
+    return f;
+}
+
+template <class Loader>
+__global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams P, float* out) {
+    const float fr = (float)(P.R - 1);
+    const float Rf = (float)P.R;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        float x, y, z;
+        double dx, dy, dz;
+        load(i, x, y, z, dx, dy, dz);
+        if (P.mode == ITS_EVAL_SUPPORT) {
+            if (dx <= P.blo[0] || dy <= P.blo[1] || dz <= P.blo[2] || dx >= P.bhi[0] || dy >= P.bhi[1] ||
+                dz >= P.bhi[2]) {
+                out[i] = P.out_val;
+                continue;
+            }
+        } else if (P.mode == ITS_EVAL_DENSE) {
+            if (dx < P.blo[0] || dy < P.blo[1] || dz < P.blo[2] || dx > P.bhi[0] || dy > P.bhi[1] || dz > P.bhi[2]) {
+                out[i] = P.out_val;
+                continue;
+            }
+        }
+        const float ux = (x - P.ox) * P.inv_w, uy = (y - P.oy) * P.inv_w, uz = (z - P.oz) * P.inv_w;
+        // clamp in float first: far or non-finite points stay defined
+        const int cx = (int)fminf(fmaxf(floorf(ux), 0.f), fr), cy = (int)fminf(fmaxf(floorf(uy), 0.f), fr),
+                  cz = (int)fminf(fmaxf(floorf(uz), 0.f), fr);
+        const uint32_t m0 = morton3(cx, cy, cz);
+        const bool inside = ux >= 0.f && uy >= 0.f && uz >= 0.f && ux <= Rf && uy <= Rf && uz <= Rf;
+        if (!inside) {
+            out[i] = eval_walk(P, ux, uy, uz, cx, cy, cz, m0);
+            continue;
+        }
+        // levels >= lc: one trilinear interpolation of the cell's G block.
+        // A block is one 64-byte line: 8 corner values, then the index of its
+        // first marked child and the 8-bit mask of marked children.
+        const int lc = P.lc;
+        const float sc = pow2neg(lc);
+        const int gx = cx >> lc, gy = cy >> lc, gz = cz >> lc;
+        const float4* gb = P.G + 4 * ((gz * P.Rc + gy) * P.Rc + gx);
+        const float4 g0 = gb[0], g1 = gb[1], gl = gb[2];
+        float f = tri_in(g0, g1, ux * sc - (float)gx, uy * sc - (float)gy, uz * sc - (float)gz);
+        uint32_t base = __float_as_uint(gl.x), mask = __float_as_uint(gl.y);
+        // marked fine levels lc-1 .. 0 through the child links
+        for (int d = lc - 1; d >= 0; --d) {
+            const uint32_t key = ((cx >> d) & 1) | (((cy >> d) & 1) << 1) | (((cz >> d) & 1) << 2);
+            if (!((mask >> key) & 1u)) break;
+            const uint32_t idx = base + __popc(mask & ((1u << key) - 1u));
+            float4 b0, b1;
+            if (d > 0) {
+                const float4* nb = P.nodes[d] + 4 * idx;
+                b0 = nb[0];
+                b1 = nb[1];
+                const float4 nl = nb[2];
+                base = __float_as_uint(nl.x);
+                mask = __float_as_uint(nl.y);
+            } else {
+                b0 = P.blocks[0][2 * idx];
+                b1 = P.blocks[0][2 * idx + 1];
+            }
+            const float s = pow2neg(d);
+            f += tri_in(b0, b1, ux * s - (float)(cx >> d), uy * s - (float)(cy >> d), uz * s - (float)(cz >> d));
+        }
+        out[i] = f;
+    }
+}
+
+// One level of the dense prolongation cascade on the full lattice:
+// g_d(X) = lambda_d(X) (0 where X is not a vertex) + trilinear interpolation
+// of g_{d+1} at X/2.  binary64, x-fastest grids.
+__global__ void k_dense_level(int D, const double* gc, int Dcoarse, const RankWord* lat, int64_t voff,
+                              const double* lambda, double* g) {
+    int64_t n = (int64_t)D * D * D;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int X = (int)(i % D), Y = (int)((i / D) % D), Z = (int)(i / ((int64_t)D * D));
+        double v = 0.0;
+        if (gc) {
+            int xs[2] = {X >> 1, (X + 1) >> 1}, ys[2] = {Y >> 1, (Y + 1) >> 1}, zs[2] = {Z >> 1, (Z + 1) >> 1};
+            int nx = (X & 1) ? 2 : 1, ny = (Y & 1) ? 2 : 1, nz = (Z & 1) ? 2 : 1;
+            double s = 0.0;
+            for (int c = 0; c < nz; ++c)
+                for (int b = 0; b < ny; ++b)
+                    for (int a = 0; a < nx; ++a) s += gc[((int64_t)zs[c] * Dcoarse + ys[b]) * Dcoarse + xs[a]];
+            v = s / (double)(nx * ny * nz);
+        }
+        uint32_t r = bit_find(lat, (uint32_t)(((int64_t)X * D + Y) * D + Z));
+        if (r != ITS_NONE) v += lambda[voff + r];
+        g[i] = v;
+    }
+}
+// lattice values (x fastest, D = Rc + 1 per axis) -> one 64-byte block per
+// cell: 8 corner values (index x | y<<1 | z<<2), child base, child mask
+__global__ void k_cell_blocks(int Rc, const float* lat, float* blocks) {
+    const int64_t n = (int64_t)Rc * Rc * Rc;
+    const int64_t D = Rc + 1;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        int64_t x = c % Rc, y = (c / Rc) % Rc, z = c / ((int64_t)Rc * Rc);
+        float* o = blocks + 16 * c;
+        for (int k = 0; k < 8; ++k)
+            o[k] = lat[((z + ((k >> 2) & 1)) * D + (y + ((k >> 1) & 1))) * D + (x + (k & 1))];
+        o[8] = __uint_as_float(ITS_NONE);
+        o[9] = __uint_as_float(0u);
+        for (int k = 10; k < 16; ++k) o[k] = 0.f;
+    }
+}
+
+// 32-byte coefficient blocks -> 64-byte node blocks with an empty child link
+__global__ void k_node_blocks(int64_t n, const float4* blocks, float4* nodes) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    nodes[4 * i] = blocks[2 * i];
+    nodes[4 * i + 1] = blocks[2 * i + 1];
+    nodes[4 * i + 2] = make_float4(__uint_as_float(ITS_NONE), __uint_as_float(0u), 0.f, 0.f);
+    nodes[4 * i + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// every marked cell of the child level links into its parent's block: the
+// smallest child index and the bit of its key (marked cells only have marked
+// parents, so the parent block exists)
+__global__ void k_link(int64_t n, const uint32_t* codes, const RankWord* parent_cells, int parent_dense_R,
+                       float4* parent_blocks, int* err) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = codes[i], p = c >> 3;
+    uint32_t slot;
+    if (parent_dense_R > 0) {
+        uint32_t x, y, z;
+        demorton3(p, x, y, z);
+        slot = (z * (uint32_t)parent_dense_R + y) * (uint32_t)parent_dense_R + x;
+    } else {
+        slot = bit_find(parent_cells, p);
+        if (slot == ITS_NONE) { atomicOr(err, 1); return; }
+    }
+    unsigned int* link = reinterpret_cast<unsigned int*>(parent_blocks + 4 * (int64_t)slot + 2);
+    atomicMin(&link[0], (unsigned int)i);
+    atomicOr(&link[1], 1u << (c & 7));
+}
+
+__global__ void k_emit_codes(const RankWord* g, int64_t nwords, uint32_t* out) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    RankWord rw = g[w];
+    uint32_t b = rw.bits, r = rw.rank;
+    while (b) {
+        int i = __ffs(b) - 1;
+        out[r++] = (uint32_t)(w * 32 + i);
+        b &= b - 1;
+    }
+}
+__global__ void k_to_f32(int64_t n, const double* a, float* b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
 }
 
 EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_val) {
@@ -153,8 +315,12 @@ EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_v
         P.cells[d] = s->lv[d].cells.p;
         P.blocks[d] = s->lv[d].blocks.p;
     }
+    P.G = s->dense_blocks.p;
+    for (int d = 0; d < s->depth; ++d) P.nodes[d] = s->lv[d].nodes.p;
     P.L = s->depth;
     P.R = s->R;
+    P.lc = s->dense_level;
+    P.Rc = s->lv[s->dense_level].R;
     P.ox = (float)s->origin[0];
     P.oy = (float)s->origin[1];
     P.oz = (float)s->origin[2];
@@ -173,10 +339,17 @@ EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_v
     }
     return P;
 }
+
+// persistent grid: enough workgroups to fill every CU several times over
+unsigned eval_grid(its_svo s, int64_t N) {
+    int64_t want = std::max<int64_t>(1, (int64_t)s->ctx->prop.multiProcessorCount * tune_get(ITS_TUNE_EVAL_BLOCKS_PER_CU));
+    return (unsigned)std::min<int64_t>(want, blocks_for(N, kT));
+}
 }  // namespace
 
 void field_build(its_svo s) {
     ITS_REQUIRE(s->has_lambda, ITS_E_STATE, "no coefficients");
+    ITS_REQUIRE(s->depth > 0, ITS_E_STATE, "empty octree");
     hipStream_t st = s->ctx->stream;
     for (int d = 0; d < s->depth; ++d) {
         Level& L = s->lv[d];
@@ -199,15 +372,78 @@ void field_build(its_svo s) {
                            (float*)L.blocks.p);
         ITS_LAUNCH_CHECK();
     }
-    ITS_HIP(hipStreamSynchronize(st));
+    // dense cascade from the root level down to the collapsed level
+    int lc = 0;
+    const int64_t max_cells = tune_get(ITS_TUNE_DENSE_CELLS);
+    while (lc < s->depth - 1 && (int64_t)s->lv[lc].R * s->lv[lc].R * s->lv[lc].R > max_cells) ++lc;
+    s->dense_level = lc;
+    {
+        DBuf<float> lat;
+        dense_cascade(s, lc, lat);
+        const int Rc = s->lv[lc].R;
+        const int64_t nc = (int64_t)Rc * Rc * Rc;
+        s->dense_blocks.ensure(4 * nc);
+        hipLaunchKernelGGL(k_cell_blocks, dim3((unsigned)std::min<int64_t>(blocks_for(nc, kT), 65536)), dim3(kT), 0, st,
+                           Rc, lat.p, (float*)s->dense_blocks.p);
+        ITS_LAUNCH_CHECK();
+    }
+    // node blocks with child links for the sparse levels below the dense one
+    DBuf<int> err(1);
+    ITS_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    for (int d = 1; d < lc; ++d) {
+        Level& L = s->lv[d];
+        L.nodes.ensure(std::max<int64_t>(4 * L.n_cells, 4));
+        hipLaunchKernelGGL(k_node_blocks, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, L.blocks.p,
+                           L.nodes.p);
+        ITS_LAUNCH_CHECK();
+    }
+    for (int d = 0; d < lc; ++d) {   // link the marked cells of level d into level d+1
+        Level& L = s->lv[d];
+        const int64_t nwords = std::max<int64_t>(1, ((int64_t)L.R * L.R * L.R + 31) / 32);
+        DBuf<uint32_t> codes(std::max<int64_t>(L.n_cells, 1));
+        hipLaunchKernelGGL(k_emit_codes, dim3(blocks_for(nwords, kT)), dim3(kT), 0, st, L.cells.p, nwords, codes.p);
+        ITS_LAUNCH_CHECK();
+        const bool dense_parent = (d + 1 == lc);
+        hipLaunchKernelGGL(k_link, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, codes.p,
+                           s->lv[d + 1].cells.p, dense_parent ? s->lv[lc].R : 0,
+                           dense_parent ? s->dense_blocks.p : s->lv[d + 1].nodes.p, err.p);
+        ITS_LAUNCH_CHECK();
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    ITS_REQUIRE(read1(err.p, st) == 0, ITS_E_STATE, "marked cell without a marked parent");
     s->field_ready = true;
+}
+
+// f at every lattice point of level `stop` (values of levels >= stop only
+// when stop > 0), binary32, x fastest
+void dense_cascade(its_svo s, int stop, DBuf<float>& out) {
+    hipStream_t st = s->ctx->stream;
+    DBuf<double> prev, cur;
+    int Dprev = 0;
+    for (int d = s->depth - 1; d >= stop; --d) {
+        const int D = s->lv[d].R + 1;
+        const int64_t n = (int64_t)D * D * D;
+        cur.alloc(n);
+        unsigned nb = (unsigned)std::min<int64_t>(blocks_for(n, kT), 65536);
+        hipLaunchKernelGGL(k_dense_level, dim3(nb), dim3(kT), 0, st, D, d == s->depth - 1 ? nullptr : prev.p, Dprev,
+                           s->lv[d].lattice.p, s->esum_verts[d], s->lambda64.p, cur.p);
+        ITS_LAUNCH_CHECK();
+        prev = std::move(cur);
+        Dprev = D;
+    }
+    const int64_t n = (int64_t)Dprev * Dprev * Dprev;
+    out.ensure(n);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<int64_t>(blocks_for(n, kT), 65536)), dim3(kT), 0, st, n,
+                       prev.p, out.p);
+    ITS_LAUNCH_CHECK();
+    ITS_HIP(hipStreamSynchronize(st));
 }
 
 void eval_points(its_svo s, const float* x, const float* y, const float* z, int64_t N, const double box_end[3],
                  int mode, float out_val, float* f, hipStream_t st) {
     EvalParams P = make_params(s, box_end, mode, out_val);
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_eval<SoaF32>, dim3(blocks_for(N, kT)), dim3(kT), 0, st, SoaF32{x, y, z}, N, P, f);
+    hipLaunchKernelGGL(k_eval<SoaF32>, dim3(eval_grid(s, N)), dim3(kT), 0, st, SoaF32{x, y, z}, N, P, f);
     ITS_LAUNCH_CHECK();
 }
 
@@ -215,7 +451,7 @@ void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box
                        float* f, hipStream_t st) {
     EvalParams P = make_params(s, box_end, mode, out_val);
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_eval<AosF64>, dim3(blocks_for(N, kT)), dim3(kT), 0, st, AosF64{pts}, N, P, f);
+    hipLaunchKernelGGL(k_eval<AosF64>, dim3(eval_grid(s, N)), dim3(kT), 0, st, AosF64{pts}, N, P, f);
     ITS_LAUNCH_CHECK();
 }
 
@@ -225,7 +461,7 @@ void eval_lattice(its_svo s, const double o[3], const double vs[3], const int64_
     int64_t N = dims[0] * dims[1] * dims[2];
     if (N <= 0) return;
     LatticeLoad ld{{o[0], o[1], o[2]}, {vs[0], vs[1], vs[2]}, dims[0], dims[1]};
-    hipLaunchKernelGGL(k_eval<LatticeLoad>, dim3(blocks_for(N, kT)), dim3(kT), 0, st, ld, N, P, out);
+    hipLaunchKernelGGL(k_eval<LatticeLoad>, dim3(eval_grid(s, N)), dim3(kT), 0, st, ld, N, P, out);
     ITS_LAUNCH_CHECK();
 }
 

@@ -21,6 +21,7 @@ struct Level {
     // evaluation field: cells whose corners include a vertex
     DBuf<RankWord> cells;   // R^3 bits by Morton code
     DBuf<float4> blocks;    // 2 float4 (8 corner coefficients) per marked cell
+    DBuf<float4> nodes;     // 4 float4 per marked cell: coefficients + child link
     int64_t n_cells = 0;
 };
 
@@ -66,15 +67,20 @@ struct its_svo_s {
     its::DBuf<double> lambda64;
     bool has_lambda = false;
     bool field_ready = false;
+    its::DBuf<float4> dense_blocks;   // per dense_level cell: corner values of levels >= dense_level
+    int dense_level = 0;
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};
 };
 
 namespace its {
+// tunables (capi.hip)
+int64_t tune_get(int key);
 // svo.hip
 void svo_build(its_svo s, const double* d_tris, int64_t T);
 // eval.hip
 void field_build(its_svo s);   // cells + blocks from lambda64
+void dense_cascade(its_svo s, int stop, DBuf<float>& out);
 void eval_points(its_svo s, const float* x, const float* y, const float* z, int64_t N, const double box_end[3],
                  int mode, float out_val, float* f, hipStream_t st);
 void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box_end[3], int mode, float out_val,

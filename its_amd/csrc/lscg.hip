@@ -36,6 +36,11 @@
 
 namespace its {
 
+struct Coarse {
+    int L, lc;
+    uint32_t off[kMaxLevels + 1];   // global vertex offsets
+};
+
 struct Fit {
     int64_t n = 0;
     int L = 0;
@@ -53,11 +58,12 @@ struct Fit {
     // reductions and device scalars
     DBuf<double> part;
     DBuf<double> scal;     // see Scal
-    DBuf<uint32_t> ticket;
     DBuf<int> ctl;         // [0] done, [1] iterations, [2] structure error
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_chunk = 0;
+    int lc = 0;            // first level handled by the one-workgroup coarse kernels
+    Coarse coarse{};
     ~Fit() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -168,39 +174,55 @@ __global__ void k_down(int64_t n, const uint32_t* dn1, uint32_t* down, uint8_t* 
 // values of the corners of the cell containing v_i at every coarser level
 // (the triplets of ThinShells.cpp:567-601).  Squares accumulate per column
 // in 2^-32 fixed point, so the sum is exact integer arithmetic and the result
-// is independent of the order of the atomics.
-__global__ void k_colnorm(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
-                          const uint16_t* vz, LatticeSet S, const uint32_t* dn1, unsigned long long* acc, int* err) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// is independent of the order of the atomics.  Columns of the coarse levels
+// (index >= coarse_off, at most kColLds of them) receive a contribution from
+// almost every row; a workgroup first accumulates those in LDS and adds each
+// non-zero total to memory once.
+constexpr int kColLds = 4096;
+__global__ __launch_bounds__(kT) void k_colnorm(int64_t n, const uint8_t* lev, int L, const uint16_t* vx,
+                                               const uint16_t* vy, const uint16_t* vz, LatticeSet S,
+                                               const uint32_t* dn1, unsigned long long* acc, uint32_t coarse_off,
+                                               uint32_t n_coarse, int* err) {
+    __shared__ unsigned long long sh[kColLds];
+    for (uint32_t k = threadIdx.x; k < n_coarse; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
     const unsigned long long one = 1ull << 32;
-    atomicAdd(&acc[i], one);
-    for (uint32_t k = dn1[i]; k != ITS_NONE; k = dn1[k]) atomicAdd(&acc[k], one);
-    int d = lev[i];
-    uint32_t X = vx[i], Y = vy[i], Z = vz[i];
-    for (int l = d + 1; l < L; ++l) {
-        int s = l - d;
-        uint32_t mask = (1u << s) - 1u;
-        double inv = 1.0 / (double)(1u << s);
-        uint32_t q[3] = {X >> s, Y >> s, Z >> s};
-        uint32_t rm[3] = {X & mask, Y & mask, Z & mask};
-        double hv[3][2];
-        int cnt[3];
-        for (int a = 0; a < 3; ++a) {
-            double fr = (double)rm[a] * inv;
-            hv[a][0] = 1.0 - fr;
-            hv[a][1] = fr;
-            cnt[a] = rm[a] ? 2 : 1;
+    auto add = [&](uint32_t col, unsigned long long v) {
+        if (col >= coarse_off) atomicAdd(&sh[col - coarse_off], v);
+        else atomicAdd(&acc[col], v);
+    };
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        add((uint32_t)i, one);
+        for (uint32_t k = dn1[i]; k != ITS_NONE; k = dn1[k]) add(k, one);
+        int d = lev[i];
+        uint32_t X = vx[i], Y = vy[i], Z = vz[i];
+        for (int l = d + 1; l < L; ++l) {
+            int s = l - d;
+            uint32_t mask = (1u << s) - 1u;
+            double inv = 1.0 / (double)(1u << s);
+            uint32_t q[3] = {X >> s, Y >> s, Z >> s};
+            uint32_t rm[3] = {X & mask, Y & mask, Z & mask};
+            double hv[3][2];
+            int cnt[3];
+            for (int a = 0; a < 3; ++a) {
+                double fr = (double)rm[a] * inv;
+                hv[a][0] = 1.0 - fr;
+                hv[a][1] = fr;
+                cnt[a] = rm[a] ? 2 : 1;
+            }
+            for (int c = 0; c < cnt[2]; ++c)
+                for (int b = 0; b < cnt[1]; ++b)
+                    for (int a = 0; a < cnt[0]; ++a) {
+                        uint32_t j = lat_find(S, l, q[0] + a, q[1] + b, q[2] + c);
+                        if (j == ITS_NONE) { atomicOr(err, 2); continue; }
+                        double v = hv[0][a] * hv[1][b] * hv[2][c];
+                        add(j, (unsigned long long)llrint(v * v * 4294967296.0));
+                    }
         }
-        for (int c = 0; c < cnt[2]; ++c)
-            for (int b = 0; b < cnt[1]; ++b)
-                for (int a = 0; a < cnt[0]; ++a) {
-                    uint32_t j = lat_find(S, l, q[0] + a, q[1] + b, q[2] + c);
-                    if (j == ITS_NONE) { atomicOr(err, 2); continue; }
-                    double v = hv[0][a] * hv[1][b] * hv[2][c];
-                    atomicAdd(&acc[j], (unsigned long long)llrint(v * v * 4294967296.0));
-                }
     }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n_coarse; k += blockDim.x)
+        if (sh[k]) atomicAdd(&acc[coarse_off + k], sh[k]);
 }
 __global__ void k_invdiag(int64_t n, const unsigned long long* acc, double* d64, float* d32) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,20 +266,19 @@ __device__ inline double block_sum(double v, double* sh) {
     return s;
 }
 
-// last-block-done reduction: returns true in the one block that sees all
-// partials; that block sums them in index order.
-__device__ inline bool last_block(uint32_t* ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-    __syncthreads();
-    return is_last;
+// Reductions write one partial per block; a one-workgroup kernel that
+// follows sums them (kernel boundaries order the writes, so no fences or
+// atomics are needed) in a fixed order: per-thread strided sums, then a
+// fixed tree.  The total is valid in thread 0.
+__device__ inline double sum_partials(const double* part, unsigned np, int stride, int o, double* sh) {
+    double v = 0;
+    for (unsigned b = threadIdx.x; b < np; b += blockDim.x) v += part[b * stride + o];
+    return block_sum(v, sh);
 }
 
-// t = D g, ||t||^2, then alpha = absNew / ||t||^2
+// t = D g and the partials of ||t||^2
 __global__ __launch_bounds__(kT) void k_apply_down(int64_t n, const uint32_t* down, const float* g, float* t,
-                                                  double* part, uint32_t* ticket, double* scal, const int* ctl) {
+                                                  double* part, const int* ctl) {
     if (ctl[0]) return;
     __shared__ double sh[kT / 64];
     double acc = 0;
@@ -268,14 +289,15 @@ __global__ __launch_bounds__(kT) void k_apply_down(int64_t n, const uint32_t* do
     }
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(ticket)) {
-        if (threadIdx.x == 0) {
-            double tt = 0;
-            for (unsigned b = 0; b < gridDim.x; ++b) tt += part[b];
-            scal[S_TT] = tt;
-            scal[S_ALPHA] = tt > 0 ? scal[S_ABSNEW] / tt : 0.0;
-            *ticket = 0;
-        }
+}
+// alpha = absNew / ||t||^2
+__global__ __launch_bounds__(kT) void k_scalar_alpha(const double* part, unsigned np, double* scal, const int* ctl) {
+    if (ctl[0]) return;
+    __shared__ double sh[kT / 64];
+    double tt = sum_partials(part, np, 1, 0, sh);
+    if (threadIdx.x == 0) {
+        scal[S_TT] = tt;
+        scal[S_ALPHA] = tt > 0 ? scal[S_ABSNEW] / tt : 0.0;
     }
 }
 
@@ -314,9 +336,58 @@ __global__ __launch_bounds__(kT) void k_restrict(int64_t b, int64_t e, const uin
     h[c] += s;
 }
 
-// z = M^-1 nr; rn2 = nr.nr; absNew' = nr.z; convergence test; beta
+// The coarse levels [lc, L) hold few vertices (at most kCoarse).  One
+// workgroup runs all of them with the level values in LDS and a barrier
+// between levels, replacing one launch per level.
+constexpr int kCoarse = 12288;
+constexpr int kCoarseT = 1024;
+__global__ __launch_bounds__(kCoarseT) void k_prolong_coarse(Coarse C, const uint32_t* ptr, const uint32_t* idx,
+                                                            float* p, const float* z, float* g, const double* scal,
+                                                            const int* ctl, int upd) {
+    if (upd && ctl[0]) return;
+    __shared__ float sg[kCoarse];
+    const uint32_t base = C.off[C.lc];
+    const float beta = upd ? (float)scal[S_BETA] : 0.f;
+    for (int d = C.L - 1; d >= C.lc; --d) {
+        for (uint32_t i = C.off[d] + threadIdx.x; i < C.off[d + 1]; i += blockDim.x) {
+            float pi = p[i];
+            if (upd) { pi = z[i] + beta * pi; p[i] = pi; }
+            uint32_t k0 = ptr[i], k1 = ptr[i + 1];
+            float s = 0.f;
+            for (uint32_t k = k0; k < k1; ++k) s += sg[idx[k] - base];
+            float w = (k1 > k0) ? 1.f / (float)(k1 - k0) : 0.f;
+            float v = pi + w * s;
+            sg[i - base] = v;
+            g[i] = v;
+        }
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(kCoarseT) void k_restrict_coarse(Coarse C, const uint32_t* ptr, const uint32_t* ridx,
+                                                             float* h, const int* ctl, int check) {
+    if (check && ctl[0]) return;
+    __shared__ float sh[kCoarse];
+    const uint32_t base = C.off[C.lc];
+    for (int d = C.lc; d < C.L; ++d) {
+        for (uint32_t c = C.off[d] + threadIdx.x; c < C.off[d + 1]; c += blockDim.x) {
+            float s = 0.f;
+            for (uint32_t k = ptr[c]; k < ptr[c + 1]; ++k) {
+                uint32_t v = ridx[k];
+                uint32_t y = v & 0x0FFFFFFFu;
+                float hy = (d == C.lc) ? h[y] : sh[y - base];
+                s += (1.f / (float)(1u << (v >> 28))) * hy;
+            }
+            float hc = h[c] + s;
+            sh[c - base] = hc;
+            h[c] = hc;
+        }
+        __syncthreads();
+    }
+}
+
+// z = M^-1 nr and the partials of nr.nr and nr.z
 __global__ __launch_bounds__(kT) void k_precond(int64_t n, const float* nr, const float* invd, float* z, double* part,
-                                               uint32_t* ticket, double* scal, int* ctl, int init) {
+                                               const int* ctl) {
     if (ctl[0]) return;
     __shared__ double sh[kT / 64];
     double a = 0, c = 0;
@@ -333,30 +404,29 @@ __global__ __launch_bounds__(kT) void k_precond(int64_t n, const float* nr, cons
         part[2 * blockIdx.x] = sa;
         part[2 * blockIdx.x + 1] = sc;
     }
-    if (last_block(ticket)) {
-        if (threadIdx.x == 0) {
-            double rn2 = 0, zz = 0;
-            for (unsigned b = 0; b < gridDim.x; ++b) {
-                rn2 += part[2 * b];
-                zz += part[2 * b + 1];
-            }
-            scal[S_RN2] = rn2;
-            if (init) {
-                scal[S_RN2_0] = rn2;
-                scal[S_ABSNEW] = zz;
-                scal[S_BETA] = 0.0;
-                if (rn2 < scal[S_THRESH] || zz == 0.0) ctl[0] = 1;
+}
+// rn2, absNew', the stopping test of Eigen's loop, beta
+__global__ __launch_bounds__(kT) void k_scalar_beta(const double* part, unsigned np, double* scal, int* ctl, int init) {
+    if (ctl[0]) return;
+    __shared__ double sh[kT / 64];
+    double rn2 = sum_partials(part, np, 2, 0, sh);
+    double zz = sum_partials(part, np, 2, 1, sh);
+    if (threadIdx.x == 0) {
+        scal[S_RN2] = rn2;
+        if (init) {
+            scal[S_RN2_0] = rn2;
+            scal[S_ABSNEW] = zz;
+            scal[S_BETA] = 0.0;
+            if (rn2 < scal[S_THRESH] || zz == 0.0) ctl[0] = 1;
+        } else {
+            ctl[1] += 1;
+            if (rn2 < scal[S_THRESH] || ctl[1] >= ctl[3]) {
+                ctl[0] = 1;
             } else {
-                ctl[1] += 1;
-                if (rn2 < scal[S_THRESH] || ctl[1] >= ctl[3]) {
-                    ctl[0] = 1;
-                } else {
-                    double absOld = scal[S_ABSNEW];
-                    scal[S_ABSNEW] = zz;
-                    scal[S_BETA] = zz / absOld;
-                }
+                double absOld = scal[S_ABSNEW];
+                scal[S_ABSNEW] = zz;
+                scal[S_BETA] = zz / absOld;
             }
-            *ticket = 0;
         }
     }
 }
@@ -386,22 +456,18 @@ __global__ void k_axpy_f32_to64(int64_t n, const float* x, double* y) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] += (double)x[i];
 }
-__global__ __launch_bounds__(kT) void k_dot64(int64_t n, const double* a, const double* b, double* part,
-                                             uint32_t* ticket, double* out) {
+__global__ __launch_bounds__(kT) void k_dot64(int64_t n, const double* a, const double* b, double* part) {
     __shared__ double sh[kT / 64];
     double acc = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         acc += a[i] * b[i];
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(ticket)) {
-        if (threadIdx.x == 0) {
-            double t = 0;
-            for (unsigned k = 0; k < gridDim.x; ++k) t += part[k];
-            *out = t;
-            *ticket = 0;
-        }
-    }
+}
+__global__ __launch_bounds__(kT) void k_sum_partials(const double* part, unsigned np, double* out) {
+    __shared__ double sh[kT / 64];
+    double t = sum_partials(part, np, 1, 0, sh);
+    if (threadIdx.x == 0) *out = t;
 }
 __global__ void k_set_thresh(double* scal, double rel2) { scal[S_THRESH] = rel2 * scal[S_RN2_0]; }
 
@@ -417,6 +483,13 @@ Fit* fit_get(its_svo s) {
     f->n = n;
     f->L = L;
     for (int d = 0; d <= L; ++d) f->off[d] = s->esum_verts[d];
+    // coarse levels for the one-workgroup kernels: as many top levels as fit
+    f->lc = L;
+    const int64_t coarse_max = std::min<int64_t>(kCoarse, tune_get(ITS_TUNE_COARSE_VERTS));
+    while (f->lc > 0 && s->esum_verts[L] - s->esum_verts[f->lc - 1] <= coarse_max) --f->lc;
+    f->coarse.L = L;
+    f->coarse.lc = f->lc;
+    for (int d = 0; d <= L; ++d) f->coarse.off[d] = (uint32_t)s->esum_verts[d];
     LatticeSet S{};
     for (int d = 0; d < L; ++d) {
         S.lat[d] = s->lv[d].lattice.p;
@@ -472,8 +545,13 @@ Fit* fit_get(its_svo s) {
         // preconditioner: inverse squared column norms
         DBuf<unsigned long long> acc(n);
         ITS_HIP(hipMemsetAsync(acc.p, 0, acc.bytes(), st));
-        hipLaunchKernelGGL(k_colnorm, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
-                           s->vz.p, S, dn1.p, acc.p, err);
+        int lc = L;   // coarse levels whose columns fit the LDS accumulator
+        while (lc > 0 && s->esum_verts[L] - s->esum_verts[lc - 1] <= kColLds) --lc;
+        uint32_t coarse_off = (uint32_t)s->esum_verts[lc];
+        uint32_t n_coarse = (uint32_t)(s->esum_verts[L] - s->esum_verts[lc]);
+        unsigned nb = (unsigned)std::min<int64_t>(blocks_for(n, kT), 2048);
+        hipLaunchKernelGGL(k_colnorm, dim3(nb), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p, s->vz.p, S,
+                           dn1.p, acc.p, coarse_off, n_coarse, err);
         ITS_LAUNCH_CHECK();
         f->invd64.alloc(n);
         f->invd32.alloc(n);
@@ -487,8 +565,6 @@ Fit* fit_get(its_svo s) {
     for (auto* v : {&f->g64, &f->h64, &f->r64, &f->nr64, &f->b64, &f->tmp64}) v->alloc(n);
     f->part.alloc(2 * kRedBlocks);
     f->scal.alloc(S_COUNT);
-    f->ticket.alloc(1);
-    ITS_HIP(hipMemsetAsync(f->ticket.p, 0, sizeof(uint32_t), st));
     ITS_HIP(hipMemsetAsync(f->scal.p, 0, S_COUNT * sizeof(double), st));
     s->fit = std::unique_ptr<Fit, void (*)(Fit*)>(f.release(), fit_release);
     return s->fit.get();
@@ -522,7 +598,9 @@ void apply_AT64(Fit* f, const double* x, double* y, hipStream_t st) {
 }
 double dot64(Fit* f, const double* a, const double* b, hipStream_t st) {
     unsigned nb = red_blocks(f->n);
-    hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kT), 0, st, f->n, a, b, f->part.p, f->ticket.p, f->scal.p + S_ZZ);
+    hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kT), 0, st, f->n, a, b, f->part.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p + S_ZZ);
     ITS_LAUNCH_CHECK();
     return read1(f->scal.p + S_ZZ, st);
 }
@@ -532,19 +610,35 @@ void launch_AT32(Fit* f, hipStream_t st, int do_update, int check) {
     hipLaunchKernelGGL(k_update_dt, dim3(blocks_for(f->n, kT)), dim3(kT), 0, st, f->n, f->finest.p, f->up.p, f->x.p,
                        f->p.p, f->r.p, f->t.p, f->h.p, f->scal.p, f->ctl.p, do_update);
     ITS_LAUNCH_CHECK();
-    for (int d = 1; d < f->L; ++d) {
+    for (int d = 1; d < f->lc; ++d) {
         int64_t b = f->off[d], e = f->off[d + 1];
         if (e <= b) continue;
         hipLaunchKernelGGL((k_restrict<float>), dim3(blocks_for(e - b, kT)), dim3(kT), 0, st, b, e, f->res_ptr.p,
                            f->res_idx.p, f->h.p, f->ctl.p, check);
         ITS_LAUNCH_CHECK();
     }
+    if (f->lc < f->L) {
+        // level 0 has no restriction: a coarse range starting at 0 restricts from level 1 on
+        Coarse C = f->coarse;
+        if (C.lc == 0) C.lc = 1;
+        if (C.lc < C.L) {
+            hipLaunchKernelGGL(k_restrict_coarse, dim3(1), dim3(kCoarseT), 0, st, C, f->res_ptr.p, f->res_idx.p, f->h.p,
+                               f->ctl.p, check);
+            ITS_LAUNCH_CHECK();
+        }
+    }
 }
 
 // one CGLS iteration (all kernels early-out once ctl[0] is set)
 void launch_iteration(Fit* f, hipStream_t st) {
     const int64_t n = f->n;
-    for (int d = f->L - 1; d >= 0; --d) {
+    const unsigned nb = red_blocks(n);
+    if (f->lc < f->L) {
+        hipLaunchKernelGGL(k_prolong_coarse, dim3(1), dim3(kCoarseT), 0, st, f->coarse, f->pro_ptr.p, f->pro_idx.p,
+                           f->p.p, (const float*)f->z.p, f->g.p, (const double*)f->scal.p, (const int*)f->ctl.p, 1);
+        ITS_LAUNCH_CHECK();
+    }
+    for (int d = f->lc - 1; d >= 0; --d) {
         int64_t b = f->off[d], e = f->off[d + 1];
         if (e <= b) continue;
         hipLaunchKernelGGL((k_prolong<float, true>), dim3(blocks_for(e - b, kT)), dim3(kT), 0, st, b, e, f->pro_ptr.p,
@@ -552,12 +646,14 @@ void launch_iteration(Fit* f, hipStream_t st) {
                            (const int*)f->ctl.p);
         ITS_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_apply_down, dim3(red_blocks(n)), dim3(kT), 0, st, n, f->down.p, f->g.p, f->t.p, f->part.p,
-                       f->ticket.p, f->scal.p, f->ctl.p);
+    hipLaunchKernelGGL(k_apply_down, dim3(nb), dim3(kT), 0, st, n, f->down.p, f->g.p, f->t.p, f->part.p, f->ctl.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scalar_alpha, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p, f->ctl.p);
     ITS_LAUNCH_CHECK();
     launch_AT32(f, st, 1, 1);
-    hipLaunchKernelGGL(k_precond, dim3(red_blocks(n)), dim3(kT), 0, st, n, f->h.p, f->invd32.p, f->z.p, f->part.p,
-                       f->ticket.p, f->scal.p, f->ctl.p, 0);
+    hipLaunchKernelGGL(k_precond, dim3(nb), dim3(kT), 0, st, n, f->h.p, f->invd32.p, f->z.p, f->part.p, f->ctl.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scalar_beta, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p, f->ctl.p, 0);
     ITS_LAUNCH_CHECK();
 }
 
@@ -576,7 +672,10 @@ int64_t inner_cgls(Fit* f, const double* rhs, double rel_tol, int64_t max_it, hi
     // z = M nr, absNew, rn2_0 (threshold not yet known: set +inf-free 0 first)
     ITS_HIP(hipMemsetAsync(f->scal.p + S_THRESH, 0, sizeof(double), st));
     hipLaunchKernelGGL(k_precond, dim3(red_blocks(n)), dim3(kT), 0, st, n, f->h.p, f->invd32.p, f->z.p, f->part.p,
-                       f->ticket.p, f->scal.p, f->ctl.p, 1);
+                       f->ctl.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scalar_beta, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, red_blocks(n), f->scal.p,
+                       f->ctl.p, 1);
     ITS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_set_thresh, dim3(1), dim3(1), 0, st, f->scal.p, rel_tol * rel_tol);
     ITS_LAUNCH_CHECK();

@@ -14,6 +14,7 @@
 // the lerp fma(t, v1, fma(-t, v0, v0)).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 
 #include "internal.hpp"
 #include "mc_tables.inc"
@@ -124,13 +125,35 @@ void upload_tables() {
 }
 }  // namespace
 
-void mc_lattice(its_svo s, const double origin[3], const double width[3], const int res[3], float* h_vals) {
-    hipStream_t st = s->ctx->stream;
+// The MC grid is the octree's level-0 lattice when it has R cells per axis
+// over the octree box (the case of ThinShells::mcVisualization at the octree
+// resolution).  Its lattice values are then the dense prolongation cascade
+// down to level 0.  Otherwise every lattice point is evaluated on its own.
+// The octree lattice spacing is width.x / R on all axes, and the MC spacing is
+// width[a] / res; the float casts of the box padding make these differ by
+// O(1e-8) relative, a difference in f far below binary32 resolution.
+static void lattice_values(its_svo s, const double origin[3], const double width[3], const int res[3],
+                           DBuf<float>& lat, hipStream_t st) {
     double vs[3] = {width[0] / res[0], width[1] / res[1], width[2] / res[2]};
     int64_t dims[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
-    int64_t np = dims[0] * dims[1] * dims[2];
-    DBuf<float> lat(np);
+    bool aligned = s->depth > 0;
+    for (int a = 0; a < 3; ++a)
+        aligned = aligned && res[a] == s->R && origin[a] == s->origin[a] &&
+                  std::fabs(vs[a] - s->voxel_width) <= 1e-6 * s->voxel_width;
+    if (aligned) {
+        dense_cascade(s, 0, lat);
+        return;
+    }
+    lat.ensure(dims[0] * dims[1] * dims[2]);
     eval_lattice(s, origin, vs, dims, lat.p, st);
+}
+
+void mc_lattice(its_svo s, const double origin[3], const double width[3], const int res[3], float* h_vals) {
+    hipStream_t st = s->ctx->stream;
+    int64_t dims[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
+    int64_t np = dims[0] * dims[1] * dims[2];
+    DBuf<float> lat;
+    lattice_values(s, origin, width, res, lat, st);
     ITS_HIP(hipMemcpyAsync(h_vals, lat.p, np * sizeof(float), hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
 }
@@ -144,10 +167,9 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     auto t0 = std::chrono::steady_clock::now();
     double vs[3] = {width[0] / res[0], width[1] / res[1], width[2] / res[2]};   // MarchingCubes.cu:682
     int64_t dims[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
-    int64_t np = dims[0] * dims[1] * dims[2];
     int64_t nvox = (int64_t)res[0] * res[1] * res[2];
-    DBuf<float> lat(np);
-    eval_lattice(s, origin, vs, dims, lat.p, st);
+    DBuf<float> lat;
+    lattice_values(s, origin, width, res, lat, st);
     ITS_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     McDims D{res[0], res[1], res[2], dims[0], dims[1], (float)iso};
