@@ -173,14 +173,21 @@ int its_mc_lattice(its_ctx ctx, its_svo svo, const double origin[3], const doubl
  * ITS_TUNE_DENSE_CELLS: the evaluation field collapses every level whose
  *   cell count is at most this into one dense block grid (default 32768).
  * ITS_TUNE_EVAL_BLOCKS_PER_CU: persistent evaluation grid, workgroups per CU
- *   (default 8).
+ *   (default 64, at most 64).
  * ITS_TUNE_COARSE_VERTS: the LSCG cascade runs the top levels holding at most
  *   this many vertices in one workgroup (default 2048, at most 12288).
+ * ITS_TUNE_EVAL_ILP: queries carried per evaluation thread, 1, 2 or 4
+ *   (default 2), when the queue is off.
+ * ITS_TUNE_EVAL_QUEUE: 1 = queries that need the fine octree levels are
+ *   gathered per wavefront in LDS and walked 64 at a time; 0 = every thread
+ *   walks its own query (default 0).
  * No reference counterpart (the reference hard-codes its launch shapes). */
 #define ITS_TUNE_DENSE_CELLS 0
 #define ITS_TUNE_EVAL_BLOCKS_PER_CU 1
 #define ITS_TUNE_COARSE_VERTS 2
-#define ITS_TUNE_COUNT 3
+#define ITS_TUNE_EVAL_ILP 3
+#define ITS_TUNE_EVAL_QUEUE 4
+#define ITS_TUNE_COUNT 5
 int its_tune_set(int key, int64_t value);
 int its_tune_get(int key, int64_t* value);
 

@@ -18,7 +18,7 @@ ITS_OK = 0
 ITS_NONE = 0xFFFFFFFF
 ITS_MAX_LEVELS = 12
 EVAL_SUPPORT, EVAL_DENSE, EVAL_RAW = 0, 1, 2
-TUNE_DENSE_CELLS, TUNE_EVAL_BLOCKS_PER_CU, TUNE_COARSE_VERTS = 0, 1, 2
+TUNE_DENSE_CELLS, TUNE_EVAL_BLOCKS_PER_CU, TUNE_COARSE_VERTS, TUNE_EVAL_ILP, TUNE_EVAL_QUEUE = 0, 1, 2, 3, 4
 
 
 class LibraryMissing(RuntimeError):

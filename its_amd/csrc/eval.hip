@@ -30,6 +30,7 @@
 // per-level walk over all levels remains for points outside the box (DENSE
 // and RAW modes).
 #include <algorithm>
+#include <cmath>
 
 #include "internal.hpp"
 
@@ -83,18 +84,38 @@ struct EvalParams {
     int Rc;           // cells per axis of G (R_lc)
     float ox, oy, oz, inv_w;
     double blo[3], bhi[3];   // accept box: open (SUPPORT) or closed (DENSE)
+    float flo[3], fhi[3];    // the same bounds as binary32 (see SoaF32::in_box)
     int mode;
     float out_val;
 };
 
-// fine levels evaluated with all bitset words and blocks issued together
-constexpr int kFine = 6;
+// accept-box test in binary64 (the reference compares V3d coordinates)
+__device__ inline bool box64(const EvalParams& P, double dx, double dy, double dz) {
+    if (P.mode == ITS_EVAL_SUPPORT)
+        return !(dx <= P.blo[0] || dy <= P.blo[1] || dz <= P.blo[2] || dx >= P.bhi[0] || dy >= P.bhi[1] ||
+                 dz >= P.bhi[2]);
+    if (P.mode == ITS_EVAL_DENSE)
+        return !(dx < P.blo[0] || dy < P.blo[1] || dz < P.blo[2] || dx > P.bhi[0] || dy > P.bhi[1] || dz > P.bhi[2]);
+    return true;
+}
 
 struct SoaF32 {
     const float *x, *y, *z;
     __device__ void operator()(int64_t i, float& a, float& b, float& c, double& da, double& db, double& dc) const {
-        a = x[i]; b = y[i]; c = z[i];
-        da = a; db = b; dc = c;
+        a = x[i];
+        b = y[i];
+        c = z[i];
+        da = db = dc = 0.0;
+    }
+    // For binary32 inputs the binary64 comparisons are exact in binary32
+    // against bounds rounded outward on the host (flo/fhi).
+    __device__ bool in_box(const EvalParams& P, float x, float y, float z, double, double, double) const {
+        if (P.mode == ITS_EVAL_SUPPORT)
+            return !(x <= P.flo[0] || y <= P.flo[1] || z <= P.flo[2] || x >= P.fhi[0] || y >= P.fhi[1] ||
+                     z >= P.fhi[2]);
+        if (P.mode == ITS_EVAL_DENSE)
+            return !(x < P.flo[0] || y < P.flo[1] || z < P.flo[2] || x > P.fhi[0] || y > P.fhi[1] || z > P.fhi[2]);
+        return true;
     }
 };
 struct AosF64 {
@@ -102,6 +123,9 @@ struct AosF64 {
     __device__ void operator()(int64_t i, float& a, float& b, float& c, double& da, double& db, double& dc) const {
         da = p[3 * i]; db = p[3 * i + 1]; dc = p[3 * i + 2];
         a = (float)da; b = (float)db; c = (float)dc;
+    }
+    __device__ bool in_box(const EvalParams& P, float, float, float, double dx, double dy, double dz) const {
+        return box64(P, dx, dy, dz);
     }
 };
 
@@ -115,6 +139,9 @@ struct LatticeLoad {
         db = o[1] + (double)iy * vs[1];
         dc = o[2] + (double)iz * vs[2];
         a = (float)da; b = (float)db; c = (float)dc;
+    }
+    __device__ bool in_box(const EvalParams& P, float, float, float, double dx, double dy, double dz) const {
+        return box64(P, dx, dy, dz);
     }
 };
 
@@ -157,67 +184,247 @@ __device__ float eval_walk(const EvalParams& P, float ux, float uy, float uz, in
     return f;
 }
 
-template <class Loader>
+// Query loop.  Each thread carries K queries in lockstep (strided by the
+// grid size, so every load stays coalesced across the wavefront), which puts
+// K independent block loads in flight per level.
+template <class Loader, int K>
 __global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams P, float* out) {
     const float fr = (float)(P.R - 1);
     const float Rf = (float)P.R;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-        float x, y, z;
-        double dx, dy, dz;
-        load(i, x, y, z, dx, dy, dz);
-        if (P.mode == ITS_EVAL_SUPPORT) {
-            if (dx <= P.blo[0] || dy <= P.blo[1] || dz <= P.blo[2] || dx >= P.bhi[0] || dy >= P.bhi[1] ||
-                dz >= P.bhi[2]) {
-                out[i] = P.out_val;
+    const int lc = P.lc;
+    const float sc = pow2neg(lc);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < N; i0 += K * stride) {
+        float ux[K], uy[K], uz[K], f[K];
+        int cx[K], cy[K], cz[K];
+        bool act[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int64_t i = i0 + j * stride;
+            act[j] = false;
+            f[j] = 0.f;
+            cx[j] = cy[j] = cz[j] = 0;
+            ux[j] = uy[j] = uz[j] = 0.f;
+            if (i >= N) continue;
+            float x, y, z;
+            double dx, dy, dz;
+            load(i, x, y, z, dx, dy, dz);
+            if (!load.in_box(P, x, y, z, dx, dy, dz)) {
+                f[j] = P.out_val;
                 continue;
             }
-        } else if (P.mode == ITS_EVAL_DENSE) {
-            if (dx < P.blo[0] || dy < P.blo[1] || dz < P.blo[2] || dx > P.bhi[0] || dy > P.bhi[1] || dz > P.bhi[2]) {
-                out[i] = P.out_val;
+            ux[j] = (x - P.ox) * P.inv_w;
+            uy[j] = (y - P.oy) * P.inv_w;
+            uz[j] = (z - P.oz) * P.inv_w;
+            // clamp in float first: far or non-finite points stay defined
+            cx[j] = (int)fminf(fmaxf(floorf(ux[j]), 0.f), fr);
+            cy[j] = (int)fminf(fmaxf(floorf(uy[j]), 0.f), fr);
+            cz[j] = (int)fminf(fmaxf(floorf(uz[j]), 0.f), fr);
+            const bool inside = ux[j] >= 0.f && uy[j] >= 0.f && uz[j] >= 0.f && ux[j] <= Rf && uy[j] <= Rf &&
+                                uz[j] <= Rf;
+            if (!inside) {
+                f[j] = eval_walk(P, ux[j], uy[j], uz[j], cx[j], cy[j], cz[j], morton3(cx[j], cy[j], cz[j]));
                 continue;
             }
+            act[j] = true;
         }
-        const float ux = (x - P.ox) * P.inv_w, uy = (y - P.oy) * P.inv_w, uz = (z - P.oz) * P.inv_w;
-        // clamp in float first: far or non-finite points stay defined
-        const int cx = (int)fminf(fmaxf(floorf(ux), 0.f), fr), cy = (int)fminf(fmaxf(floorf(uy), 0.f), fr),
-                  cz = (int)fminf(fmaxf(floorf(uz), 0.f), fr);
-        const uint32_t m0 = morton3(cx, cy, cz);
-        const bool inside = ux >= 0.f && uy >= 0.f && uz >= 0.f && ux <= Rf && uy <= Rf && uz <= Rf;
-        if (!inside) {
-            out[i] = eval_walk(P, ux, uy, uz, cx, cy, cz, m0);
-            continue;
+        // levels >= lc: one 64-byte G block per query (8 corner values, then
+        // the index of the first marked child and the 8-bit child mask)
+        float4 b0[K], b1[K], bl[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (!act[j]) continue;
+            const float4* gb = P.G + 4 * (((cz[j] >> lc) * P.Rc + (cy[j] >> lc)) * P.Rc + (cx[j] >> lc));
+            b0[j] = gb[0];
+            b1[j] = gb[1];
+            bl[j] = gb[2];
         }
-        // levels >= lc: one trilinear interpolation of the cell's G block.
-        // A block is one 64-byte line: 8 corner values, then the index of its
-        // first marked child and the 8-bit mask of marked children.
-        const int lc = P.lc;
-        const float sc = pow2neg(lc);
-        const int gx = cx >> lc, gy = cy >> lc, gz = cz >> lc;
-        const float4* gb = P.G + 4 * ((gz * P.Rc + gy) * P.Rc + gx);
-        const float4 g0 = gb[0], g1 = gb[1], gl = gb[2];
-        float f = tri_in(g0, g1, ux * sc - (float)gx, uy * sc - (float)gy, uz * sc - (float)gz);
-        uint32_t base = __float_as_uint(gl.x), mask = __float_as_uint(gl.y);
+        uint32_t base[K], mask[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            base[j] = mask[j] = 0;
+            if (!act[j]) continue;
+            f[j] = tri_in(b0[j], b1[j], ux[j] * sc - (float)(cx[j] >> lc), uy[j] * sc - (float)(cy[j] >> lc),
+                          uz[j] * sc - (float)(cz[j] >> lc));
+            base[j] = __float_as_uint(bl[j].x);
+            mask[j] = __float_as_uint(bl[j].y);
+        }
         // marked fine levels lc-1 .. 0 through the child links
         for (int d = lc - 1; d >= 0; --d) {
-            const uint32_t key = ((cx >> d) & 1) | (((cy >> d) & 1) << 1) | (((cz >> d) & 1) << 2);
-            if (!((mask >> key) & 1u)) break;
-            const uint32_t idx = base + __popc(mask & ((1u << key) - 1u));
-            float4 b0, b1;
-            if (d > 0) {
-                const float4* nb = P.nodes[d] + 4 * idx;
-                b0 = nb[0];
-                b1 = nb[1];
-                const float4 nl = nb[2];
-                base = __float_as_uint(nl.x);
-                mask = __float_as_uint(nl.y);
-            } else {
-                b0 = P.blocks[0][2 * idx];
-                b1 = P.blocks[0][2 * idx + 1];
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (!act[j]) continue;
+                const uint32_t key = ((cx[j] >> d) & 1) | (((cy[j] >> d) & 1) << 1) | (((cz[j] >> d) & 1) << 2);
+                if (!((mask[j] >> key) & 1u)) { act[j] = false; continue; }
+                const uint32_t idx = base[j] + __popc(mask[j] & ((1u << key) - 1u));
+                if (d > 0) {
+                    const float4* nb = P.nodes[d] + 4 * idx;
+                    b0[j] = nb[0];
+                    b1[j] = nb[1];
+                    bl[j] = nb[2];
+                } else {
+                    b0[j] = P.blocks[0][2 * idx];
+                    b1[j] = P.blocks[0][2 * idx + 1];
+                }
+                any = true;
             }
+            if (!any) break;
             const float s = pow2neg(d);
-            f += tri_in(b0, b1, ux * s - (float)(cx >> d), uy * s - (float)(cy >> d), uz * s - (float)(cz >> d));
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (!act[j]) continue;
+                f[j] += tri_in(b0[j], b1[j], ux[j] * s - (float)(cx[j] >> d), uy[j] * s - (float)(cy[j] >> d),
+                               uz[j] * s - (float)(cz[j] >> d));
+                if (d > 0) {
+                    base[j] = __float_as_uint(bl[j].x);
+                    mask[j] = __float_as_uint(bl[j].y);
+                }
+            }
         }
-        out[i] = f;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int64_t i = i0 + j * stride;
+            if (i < N) out[i] = f[j];
+        }
+    }
+}
+
+// Query loop with a per-wavefront queue.
+//
+// Random queries diverge: most stop at the collapsed level, a few walk every
+// fine level, and a wavefront runs the longest walk of its 64 lanes.  Here
+// every wavefront evaluates the collapsed level for 64 queries at a time and
+// pushes only the queries with a marked child into an LDS stack of its own.
+// Whenever the stack holds 64 entries the wavefront walks the fine levels for
+// exactly those 64, so the fine-level instructions run on full wavefronts.
+//
+// Entries are the query index within the launch (at most 2^31 queries per
+// launch), the cell coordinates u scaled to level-0 cells, the partial sum,
+// and the child link of the collapsed cell.  u is in [0, R) on this path, so
+// floor(u 2^-d) = floor(u) >> d and the local coordinate in the level-d cell
+// is fract(u 2^-d), computed exactly by v_fract_f32.  Points on the upper
+// faces (u = R) take the clamped walk with everything outside the grid.
+constexpr int kWaves = kT / 64;
+constexpr int kQueue = 128;   // at most 63 pending + 64 new entries
+
+struct WaveQueue {
+    float ux[kWaves][kQueue], uy[kWaves][kQueue], uz[kWaves][kQueue], f[kWaves][kQueue];
+    uint32_t i[kWaves][kQueue], base[kWaves][kQueue], mask[kWaves][kQueue];
+};
+
+__device__ inline void lds_fence() {
+    // the writes of some lanes are read by other lanes of the same wavefront
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// fine levels lc-1 .. 0 for one queued query
+__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint32_t base,
+                                  uint32_t mask) {
+    const int cx = (int)ux, cy = (int)uy, cz = (int)uz;   // u >= 0: truncation is floor
+    for (int d = P.lc - 1; d >= 0; --d) {
+        const uint32_t key = ((cx >> d) & 1) | (((cy >> d) & 1) << 1) | (((cz >> d) & 1) << 2);
+        if (!((mask >> key) & 1u)) break;
+        const uint32_t idx = base + __popc(mask & ((1u << key) - 1u));
+        float4 b0, b1;
+        if (d > 0) {
+            const float4* nb = P.nodes[d] + 4 * (size_t)idx;
+            b0 = nb[0];
+            b1 = nb[1];
+            const float4 bl = nb[2];
+            base = __float_as_uint(bl.x);
+            mask = __float_as_uint(bl.y);
+        } else {
+            b0 = P.blocks[0][2 * (size_t)idx];
+            b1 = P.blocks[0][2 * (size_t)idx + 1];
+        }
+        const float s = pow2neg(d);
+        f += tri_in(b0, b1, __builtin_amdgcn_fractf(ux * s), __builtin_amdgcn_fractf(uy * s),
+                    __builtin_amdgcn_fractf(uz * s));
+    }
+    return f;
+}
+
+template <class Loader>
+__global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t N, EvalParams P, float* out) {
+    __shared__ WaveQueue Q;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float Rf = (float)P.R, fr = (float)(P.R - 1);
+    const int lc = P.lc;
+    const float sc = pow2neg(lc);
+    const uint64_t below = (1ull << lane) - 1ull;
+    int qn = 0;   // entries on this wavefront's stack (uniform)
+    const int64_t step = (int64_t)gridDim.x * kT;
+    for (int64_t c = (int64_t)blockIdx.x * kT + w * 64; c < N; c += step) {
+        const int64_t i = c + lane;
+        bool push = false;
+        float f = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+        uint32_t base = 0, mask = 0;
+        if (i < N) {
+            float x, y, z;
+            double dx, dy, dz;
+            load(off + i, x, y, z, dx, dy, dz);
+            if (!load.in_box(P, x, y, z, dx, dy, dz)) {
+                f = P.out_val;
+            } else {
+                ux = (x - P.ox) * P.inv_w;
+                uy = (y - P.oy) * P.inv_w;
+                uz = (z - P.oz) * P.inv_w;
+                if (ux >= 0.f && uy >= 0.f && uz >= 0.f && ux < Rf && uy < Rf && uz < Rf) {
+                    const int gx = (int)(ux * sc), gy = (int)(uy * sc), gz = (int)(uz * sc);
+                    const float4* gb = P.G + 4 * (size_t)((gz * P.Rc + gy) * P.Rc + gx);
+                    const float4 g0 = gb[0], g1 = gb[1], gl = gb[2];
+                    f = tri_in(g0, g1, __builtin_amdgcn_fractf(ux * sc), __builtin_amdgcn_fractf(uy * sc),
+                               __builtin_amdgcn_fractf(uz * sc));
+                    base = __float_as_uint(gl.x);
+                    mask = __float_as_uint(gl.y);
+                    if (lc > 0) {
+                        const int d = lc - 1;
+                        const uint32_t key = (((int)ux >> d) & 1) | ((((int)uy >> d) & 1) << 1) |
+                                             ((((int)uz >> d) & 1) << 2);
+                        push = (mask >> key) & 1u;
+                    }
+                } else {
+                    const int cx = (int)fminf(fmaxf(floorf(ux), 0.f), fr);
+                    const int cy = (int)fminf(fmaxf(floorf(uy), 0.f), fr);
+                    const int cz = (int)fminf(fmaxf(floorf(uz), 0.f), fr);
+                    f = eval_walk(P, ux, uy, uz, cx, cy, cz, morton3(cx, cy, cz));
+                }
+            }
+        }
+        const uint64_t bal = __ballot(push);
+        if (push) {
+            const int pos = qn + __popcll(bal & below);
+            Q.ux[w][pos] = ux;
+            Q.uy[w][pos] = uy;
+            Q.uz[w][pos] = uz;
+            Q.f[w][pos] = f;
+            Q.i[w][pos] = (uint32_t)i;
+            Q.base[w][pos] = base;
+            Q.mask[w][pos] = mask;
+        } else if (i < N) {
+            out[off + i] = f;
+        }
+        qn += __popcll(bal);
+        if (qn >= 64) {
+            lds_fence();
+            const int pos = qn - 64 + lane;
+            const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.base[w][pos],
+                                      Q.mask[w][pos]);
+            out[off + Q.i[w][pos]] = r;
+            qn -= 64;
+            lds_fence();
+        }
+    }
+    if (qn > 0) {
+        lds_fence();
+        if (lane < qn) {
+            const float r = fine_walk(P, Q.ux[w][lane], Q.uy[w][lane], Q.uz[w][lane], Q.f[w][lane], Q.base[w][lane],
+                                      Q.mask[w][lane]);
+            out[off + Q.i[w][lane]] = r;
+        }
     }
 }
 
@@ -336,6 +543,21 @@ EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_v
             P.blo[a] = s->origin[a];
             P.bhi[a] = box_end[a];
         }
+        // For a binary32 x: (x <= lo) iff (x <= largest float <= lo),
+        // (x >= hi) iff (x >= smallest float >= hi), (x < lo) iff
+        // (x < smallest float >= lo), (x > hi) iff (x > largest float <= hi).
+        float lo_dn = (float)P.blo[a], lo_up = (float)P.blo[a], hi_dn = (float)P.bhi[a], hi_up = (float)P.bhi[a];
+        if ((double)lo_dn > P.blo[a]) lo_dn = std::nextafter(lo_dn, -INFINITY);
+        if ((double)lo_up < P.blo[a]) lo_up = std::nextafter(lo_up, INFINITY);
+        if ((double)hi_dn > P.bhi[a]) hi_dn = std::nextafter(hi_dn, -INFINITY);
+        if ((double)hi_up < P.bhi[a]) hi_up = std::nextafter(hi_up, INFINITY);
+        if (mode == ITS_EVAL_DENSE) {
+            P.flo[a] = lo_up;
+            P.fhi[a] = hi_dn;
+        } else {
+            P.flo[a] = lo_dn;
+            P.fhi[a] = hi_up;
+        }
     }
     return P;
 }
@@ -344,6 +566,24 @@ EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_v
 unsigned eval_grid(its_svo s, int64_t N) {
     int64_t want = std::max<int64_t>(1, (int64_t)s->ctx->prop.multiProcessorCount * tune_get(ITS_TUNE_EVAL_BLOCKS_PER_CU));
     return (unsigned)std::min<int64_t>(want, blocks_for(N, kT));
+}
+template <class Loader>
+void launch_eval(Loader ld, its_svo s, int64_t N, const EvalParams& P, float* out, hipStream_t st) {
+    if (tune_get(ITS_TUNE_EVAL_QUEUE)) {
+        constexpr int64_t kSeg = int64_t(1) << 31;   // queue entries hold 31-bit indices
+        for (int64_t off = 0; off < N; off += kSeg) {
+            const int64_t n = std::min(kSeg, N - off);
+            hipLaunchKernelGGL(k_eval_q<Loader>, dim3(eval_grid(s, n)), dim3(kT), 0, st, ld, off, n, P, out);
+            ITS_LAUNCH_CHECK();
+        }
+        return;
+    }
+    const int64_t ilp = tune_get(ITS_TUNE_EVAL_ILP);
+    const unsigned g = eval_grid(s, (N + ilp - 1) / ilp);
+    if (ilp >= 4) hipLaunchKernelGGL((k_eval<Loader, 4>), dim3(g), dim3(kT), 0, st, ld, N, P, out);
+    else if (ilp >= 2) hipLaunchKernelGGL((k_eval<Loader, 2>), dim3(g), dim3(kT), 0, st, ld, N, P, out);
+    else hipLaunchKernelGGL((k_eval<Loader, 1>), dim3(g), dim3(kT), 0, st, ld, N, P, out);
+    ITS_LAUNCH_CHECK();
 }
 }  // namespace
 
@@ -443,16 +683,14 @@ void eval_points(its_svo s, const float* x, const float* y, const float* z, int6
                  int mode, float out_val, float* f, hipStream_t st) {
     EvalParams P = make_params(s, box_end, mode, out_val);
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_eval<SoaF32>, dim3(eval_grid(s, N)), dim3(kT), 0, st, SoaF32{x, y, z}, N, P, f);
-    ITS_LAUNCH_CHECK();
+    launch_eval(SoaF32{x, y, z}, s, N, P, f, st);
 }
 
 void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box_end[3], int mode, float out_val,
                        float* f, hipStream_t st) {
     EvalParams P = make_params(s, box_end, mode, out_val);
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_eval<AosF64>, dim3(eval_grid(s, N)), dim3(kT), 0, st, AosF64{pts}, N, P, f);
-    ITS_LAUNCH_CHECK();
+    launch_eval(AosF64{pts}, s, N, P, f, st);
 }
 
 void eval_lattice(its_svo s, const double o[3], const double vs[3], const int64_t dims[3], float* out, hipStream_t st) {
@@ -461,8 +699,7 @@ void eval_lattice(its_svo s, const double o[3], const double vs[3], const int64_
     int64_t N = dims[0] * dims[1] * dims[2];
     if (N <= 0) return;
     LatticeLoad ld{{o[0], o[1], o[2]}, {vs[0], vs[1], vs[2]}, dims[0], dims[1]};
-    hipLaunchKernelGGL(k_eval<LatticeLoad>, dim3(eval_grid(s, N)), dim3(kT), 0, st, ld, N, P, out);
-    ITS_LAUNCH_CHECK();
+    launch_eval(ld, s, N, P, out, st);
 }
 
 }  // namespace its

@@ -107,3 +107,50 @@ def test_eval_empty_and_device_path(ctx):
     g = _gpu(ctx, F)
     c = F.case
     assert len(g.eval(np.zeros((0, 3), np.float32), c.end)) == 0
+
+
+@pytest.fixture
+def restore_tunables():
+    from its_amd import _capi
+    keys = (_capi.TUNE_DENSE_CELLS, _capi.TUNE_EVAL_ILP, _capi.TUNE_EVAL_QUEUE, _capi.TUNE_EVAL_BLOCKS_PER_CU)
+    old = {k: _capi.tune_get(k) for k in keys}
+    yield
+    for k, v in old.items():
+        _capi.tune_set(k, v)
+
+
+@pytest.mark.parametrize("dense_cells", [8, 512, 32768])
+def test_eval_schemes_agree(ctx, restore_tunables, dense_cells):
+    """Every launch scheme (per-thread walk with 1, 2 or 4 queries per thread,
+    or the per-wavefront queue) and every collapse level gives the oracle's f.
+    Off the upper box faces the schemes do the same binary32 arithmetic in the
+    same order, so they agree bit for bit."""
+    from its_amd import _capi
+    F = fitted("bunny", 64)
+    c = F.case
+    _capi.tune_set(_capi.TUNE_DENSE_CELLS, dense_cells)
+    g = _gpu(ctx, F)
+    rng = np.random.default_rng(11)
+    q = rng.uniform(c.origin - 0.05, c.end + 0.05, size=(30011, 3)).astype(np.float32)
+    # points on the upper faces and corners of the grid box (u = R)
+    face = rng.uniform(c.origin, c.end, size=(64, 3)).astype(np.float32)
+    face[:, 0] = np.float32(c.end[0])
+    face[:8] = np.float32(c.end)
+    q = np.concatenate([q, face])
+    fr = F.svo.eval_dense(F.lam, q[:2000].astype(np.float64))
+    fr_face = F.svo.eval_dense(F.lam, face.astype(np.float64))
+    ref = None
+    for queue, ilp in [(0, 1), (0, 2), (0, 4), (1, 1)]:
+        _capi.tune_set(_capi.TUNE_EVAL_QUEUE, queue)
+        _capi.tune_set(_capi.TUNE_EVAL_ILP, ilp)
+        for bpc in (1, 32):
+            _capi.tune_set(_capi.TUNE_EVAL_BLOCKS_PER_CU, bpc)
+            f = g.eval(q, c.end, _capi.EVAL_RAW)
+            print(f"[eval scheme] dense={dense_cells} queue={queue} ilp={ilp} bpc={bpc}", flush=True)
+            assert np.abs(f[:2000] - fr).max() <= TOL
+            assert np.abs(f[-64:] - fr_face).max() <= TOL
+            if ref is None:
+                ref = f
+            else:
+                assert np.array_equal(f[:-64], ref[:-64])
+    g.close()
