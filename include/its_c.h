@@ -177,7 +177,7 @@ int its_mc_lattice(its_ctx ctx, its_svo svo, const double origin[3], const doubl
  * ITS_TUNE_COARSE_VERTS: the LSCG cascade runs the top levels holding at most
  *   this many vertices in one workgroup (default 2048, at most 12288).
  * ITS_TUNE_EVAL_ILP: queries carried per evaluation thread, 1, 2 or 4
- *   (default 2), when the queue is off.
+ *   (default 1), when the queue is off.
  * ITS_TUNE_EVAL_QUEUE: 1 = queries that need the fine octree levels are
  *   gathered per wavefront in LDS and walked 64 at a time; 0 = every thread
  *   walks its own query (default 0).

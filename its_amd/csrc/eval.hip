@@ -31,6 +31,7 @@
 // and RAW modes).
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -76,8 +77,9 @@ __global__ void k_fill_blocks(int64_t nv, int64_t voff, const uint16_t* vx, cons
 struct EvalParams {
     const RankWord* cells[kMaxLevels];
     const float4* blocks[kMaxLevels];
-    const float4* G;  // per level-lc cell: 8 corner values of levels >= lc + child link (4 float4)
-    const float4* nodes[kMaxLevels];   // levels 1..lc-1: coefficients + child link (4 float4)
+    const float4* G;  // per level-lc cell: 8 corner values of levels >= lc, child mask in the low bits
+    const uint32_t* Gbase;             // per level-lc cell: index of its first marked child
+    const uint32_t* cbase[kMaxLevels]; // levels 1..lc-1: index of each marked cell's first marked child
     int L;
     int R;
     int lc;           // collapsed level
@@ -184,101 +186,120 @@ __device__ float eval_walk(const EvalParams& P, float ux, float uy, float uz, in
     return f;
 }
 
+// The child mask of a cell rides in the lowest mantissa bit of its 8 corner
+// values (bit k in corner k), so one 32-byte block gives both the values and
+// the mask.  This moves each value by at most one ulp of binary32.
+__device__ inline uint32_t block_mask(float4 b0, float4 b1) {
+    return (__float_as_uint(b0.x) & 1u) | ((__float_as_uint(b0.y) & 1u) << 1) | ((__float_as_uint(b0.z) & 1u) << 2) |
+           ((__float_as_uint(b0.w) & 1u) << 3) | ((__float_as_uint(b1.x) & 1u) << 4) |
+           ((__float_as_uint(b1.y) & 1u) << 5) | ((__float_as_uint(b1.z) & 1u) << 6) |
+           ((__float_as_uint(b1.w) & 1u) << 7);
+}
+__device__ inline uint32_t child_key(int cx, int cy, int cz, int d) {
+    return ((cx >> d) & 1) | (((cy >> d) & 1) << 1) | (((cz >> d) & 1) << 2);
+}
+__device__ inline uint32_t child_index(uint32_t base, uint32_t mask, uint32_t key) {
+    return base + __popc(mask & ((1u << key) - 1u));
+}
+// local coordinates in the level-d cell: u in [0, R), so floor(u 2^-d) is
+// floor(u) >> d and the fraction is exact
+__device__ inline float tri_level(float4 b0, float4 b1, float ux, float uy, float uz, float s) {
+    return tri_in(b0, b1, __builtin_amdgcn_fractf(ux * s), __builtin_amdgcn_fractf(uy * s),
+                  __builtin_amdgcn_fractf(uz * s));
+}
+
+// The query front end shared by both launch schemes: load, accept-box test,
+// and either a finished value (outside the box, or outside [0, R) through the
+// clamped walk) or the grid coordinates for the octree walk.
+template <class Loader>
+__device__ inline bool query_front(const Loader& load, const EvalParams& P, int64_t i, float& f, float& ux, float& uy,
+                                   float& uz) {
+    float x, y, z;
+    double dx, dy, dz;
+    load(i, x, y, z, dx, dy, dz);
+    if (!load.in_box(P, x, y, z, dx, dy, dz)) {
+        f = P.out_val;
+        return false;
+    }
+    ux = (x - P.ox) * P.inv_w;
+    uy = (y - P.oy) * P.inv_w;
+    uz = (z - P.oz) * P.inv_w;
+    const float Rf = (float)P.R;
+    if (ux >= 0.f && uy >= 0.f && uz >= 0.f && ux < Rf && uy < Rf && uz < Rf) return true;
+    // clamp in float first: far or non-finite points stay defined
+    const float fr = (float)(P.R - 1);
+    const int cx = (int)fminf(fmaxf(floorf(ux), 0.f), fr);
+    const int cy = (int)fminf(fmaxf(floorf(uy), 0.f), fr);
+    const int cz = (int)fminf(fmaxf(floorf(uz), 0.f), fr);
+    f = eval_walk(P, ux, uy, uz, cx, cy, cz, morton3(cx, cy, cz));
+    return false;
+}
+
 // Query loop.  Each thread carries K queries in lockstep (strided by the
 // grid size, so every load stays coalesced across the wavefront), which puts
 // K independent block loads in flight per level.
 template <class Loader, int K>
 __global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams P, float* out) {
-    const float fr = (float)(P.R - 1);
-    const float Rf = (float)P.R;
     const int lc = P.lc;
     const float sc = pow2neg(lc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < N; i0 += K * stride) {
         float ux[K], uy[K], uz[K], f[K];
-        int cx[K], cy[K], cz[K];
         bool act[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const int64_t i = i0 + j * stride;
-            act[j] = false;
-            f[j] = 0.f;
-            cx[j] = cy[j] = cz[j] = 0;
-            ux[j] = uy[j] = uz[j] = 0.f;
-            if (i >= N) continue;
-            float x, y, z;
-            double dx, dy, dz;
-            load(i, x, y, z, dx, dy, dz);
-            if (!load.in_box(P, x, y, z, dx, dy, dz)) {
-                f[j] = P.out_val;
-                continue;
-            }
-            ux[j] = (x - P.ox) * P.inv_w;
-            uy[j] = (y - P.oy) * P.inv_w;
-            uz[j] = (z - P.oz) * P.inv_w;
-            // clamp in float first: far or non-finite points stay defined
-            cx[j] = (int)fminf(fmaxf(floorf(ux[j]), 0.f), fr);
-            cy[j] = (int)fminf(fmaxf(floorf(uy[j]), 0.f), fr);
-            cz[j] = (int)fminf(fmaxf(floorf(uz[j]), 0.f), fr);
-            const bool inside = ux[j] >= 0.f && uy[j] >= 0.f && uz[j] >= 0.f && ux[j] <= Rf && uy[j] <= Rf &&
-                                uz[j] <= Rf;
-            if (!inside) {
-                f[j] = eval_walk(P, ux[j], uy[j], uz[j], cx[j], cy[j], cz[j], morton3(cx[j], cy[j], cz[j]));
-                continue;
-            }
-            act[j] = true;
+            f[j] = ux[j] = uy[j] = uz[j] = 0.f;
+            act[j] = i < N && query_front(load, P, i, f[j], ux[j], uy[j], uz[j]);
         }
-        // levels >= lc: one 64-byte G block per query (8 corner values, then
-        // the index of the first marked child and the 8-bit child mask)
-        float4 b0[K], b1[K], bl[K];
+        // levels >= lc: one 32-byte G block per query
+        float4 b0[K], b1[K];
+        uint32_t gcell[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (!act[j]) continue;
-            const float4* gb = P.G + 4 * (((cz[j] >> lc) * P.Rc + (cy[j] >> lc)) * P.Rc + (cx[j] >> lc));
-            b0[j] = gb[0];
-            b1[j] = gb[1];
-            bl[j] = gb[2];
+            gcell[j] = (((int)(uz[j] * sc) * P.Rc + (int)(uy[j] * sc)) * P.Rc + (int)(ux[j] * sc));
+            b0[j] = P.G[2 * (size_t)gcell[j]];
+            b1[j] = P.G[2 * (size_t)gcell[j] + 1];
         }
-        uint32_t base[K], mask[K];
+        uint32_t mask[K], idx[K];
+        bool more = false;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            base[j] = mask[j] = 0;
             if (!act[j]) continue;
-            f[j] = tri_in(b0[j], b1[j], ux[j] * sc - (float)(cx[j] >> lc), uy[j] * sc - (float)(cy[j] >> lc),
-                          uz[j] * sc - (float)(cz[j] >> lc));
-            base[j] = __float_as_uint(bl[j].x);
-            mask[j] = __float_as_uint(bl[j].y);
+            f[j] = tri_level(b0[j], b1[j], ux[j], uy[j], uz[j], sc);
+            mask[j] = block_mask(b0[j], b1[j]);
+            const uint32_t key = lc > 0 ? child_key((int)ux[j], (int)uy[j], (int)uz[j], lc - 1) : 0u;
+            act[j] = lc > 0 && ((mask[j] >> key) & 1u);
+            if (act[j]) {
+                idx[j] = child_index(P.Gbase[gcell[j]], mask[j], key);
+                more = true;
+            }
         }
-        // marked fine levels lc-1 .. 0 through the child links
-        for (int d = lc - 1; d >= 0; --d) {
-            bool any = false;
+        // marked fine levels lc-1 .. 0 through the child links; the next
+        // level's first-child index is read beside the block
+        for (int d = lc - 1; more && d >= 0; --d) {
+            uint32_t nbase[K];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (!act[j]) continue;
-                const uint32_t key = ((cx[j] >> d) & 1) | (((cy[j] >> d) & 1) << 1) | (((cz[j] >> d) & 1) << 2);
-                if (!((mask[j] >> key) & 1u)) { act[j] = false; continue; }
-                const uint32_t idx = base[j] + __popc(mask[j] & ((1u << key) - 1u));
-                if (d > 0) {
-                    const float4* nb = P.nodes[d] + 4 * idx;
-                    b0[j] = nb[0];
-                    b1[j] = nb[1];
-                    bl[j] = nb[2];
-                } else {
-                    b0[j] = P.blocks[0][2 * idx];
-                    b1[j] = P.blocks[0][2 * idx + 1];
-                }
-                any = true;
+                b0[j] = P.blocks[d][2 * (size_t)idx[j]];
+                b1[j] = P.blocks[d][2 * (size_t)idx[j] + 1];
+                nbase[j] = d > 0 ? P.cbase[d][idx[j]] : 0u;
             }
-            if (!any) break;
             const float s = pow2neg(d);
+            more = false;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (!act[j]) continue;
-                f[j] += tri_in(b0[j], b1[j], ux[j] * s - (float)(cx[j] >> d), uy[j] * s - (float)(cy[j] >> d),
-                               uz[j] * s - (float)(cz[j] >> d));
-                if (d > 0) {
-                    base[j] = __float_as_uint(bl[j].x);
-                    mask[j] = __float_as_uint(bl[j].y);
+                f[j] += tri_level(b0[j], b1[j], ux[j], uy[j], uz[j], s);
+                if (d == 0) continue;
+                mask[j] = block_mask(b0[j], b1[j]);
+                const uint32_t key = child_key((int)ux[j], (int)uy[j], (int)uz[j], d - 1);
+                act[j] = (mask[j] >> key) & 1u;
+                if (act[j]) {
+                    idx[j] = child_index(nbase[j], mask[j], key);
+                    more = true;
                 }
             }
         }
@@ -300,17 +321,15 @@ __global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams 
 // exactly those 64, so the fine-level instructions run on full wavefronts.
 //
 // Entries are the query index within the launch (at most 2^31 queries per
-// launch), the cell coordinates u scaled to level-0 cells, the partial sum,
-// and the child link of the collapsed cell.  u is in [0, R) on this path, so
-// floor(u 2^-d) = floor(u) >> d and the local coordinate in the level-d cell
-// is fract(u 2^-d), computed exactly by v_fract_f32.  Points on the upper
-// faces (u = R) take the clamped walk with everything outside the grid.
+// launch), the grid coordinates u, the partial sum, the collapsed cell and
+// its child mask.  Points on the upper faces (u = R) take the clamped walk
+// with everything outside the grid.
 constexpr int kWaves = kT / 64;
 constexpr int kQueue = 128;   // at most 63 pending + 64 new entries
 
 struct WaveQueue {
     float ux[kWaves][kQueue], uy[kWaves][kQueue], uz[kWaves][kQueue], f[kWaves][kQueue];
-    uint32_t i[kWaves][kQueue], base[kWaves][kQueue], mask[kWaves][kQueue];
+    uint32_t i[kWaves][kQueue], gcell[kWaves][kQueue], mask[kWaves][kQueue];
 };
 
 __device__ inline void lds_fence() {
@@ -321,28 +340,20 @@ __device__ inline void lds_fence() {
 }
 
 // fine levels lc-1 .. 0 for one queued query
-__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint32_t base,
+__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint32_t gcell,
                                   uint32_t mask) {
     const int cx = (int)ux, cy = (int)uy, cz = (int)uz;   // u >= 0: truncation is floor
+    uint32_t idx = child_index(P.Gbase[gcell], mask, child_key(cx, cy, cz, P.lc - 1));
     for (int d = P.lc - 1; d >= 0; --d) {
-        const uint32_t key = ((cx >> d) & 1) | (((cy >> d) & 1) << 1) | (((cz >> d) & 1) << 2);
+        const float4 b0 = P.blocks[d][2 * (size_t)idx];
+        const float4 b1 = P.blocks[d][2 * (size_t)idx + 1];
+        const uint32_t nbase = d > 0 ? P.cbase[d][idx] : 0u;
+        f += tri_level(b0, b1, ux, uy, uz, pow2neg(d));
+        if (d == 0) break;
+        mask = block_mask(b0, b1);
+        const uint32_t key = child_key(cx, cy, cz, d - 1);
         if (!((mask >> key) & 1u)) break;
-        const uint32_t idx = base + __popc(mask & ((1u << key) - 1u));
-        float4 b0, b1;
-        if (d > 0) {
-            const float4* nb = P.nodes[d] + 4 * (size_t)idx;
-            b0 = nb[0];
-            b1 = nb[1];
-            const float4 bl = nb[2];
-            base = __float_as_uint(bl.x);
-            mask = __float_as_uint(bl.y);
-        } else {
-            b0 = P.blocks[0][2 * (size_t)idx];
-            b1 = P.blocks[0][2 * (size_t)idx + 1];
-        }
-        const float s = pow2neg(d);
-        f += tri_in(b0, b1, __builtin_amdgcn_fractf(ux * s), __builtin_amdgcn_fractf(uy * s),
-                    __builtin_amdgcn_fractf(uz * s));
+        idx = child_index(nbase, mask, key);
     }
     return f;
 }
@@ -351,7 +362,6 @@ template <class Loader>
 __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t N, EvalParams P, float* out) {
     __shared__ WaveQueue Q;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const float Rf = (float)P.R, fr = (float)(P.R - 1);
     const int lc = P.lc;
     const float sc = pow2neg(lc);
     const uint64_t below = (1ull << lane) - 1ull;
@@ -361,38 +371,13 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
         const int64_t i = c + lane;
         bool push = false;
         float f = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
-        uint32_t base = 0, mask = 0;
-        if (i < N) {
-            float x, y, z;
-            double dx, dy, dz;
-            load(off + i, x, y, z, dx, dy, dz);
-            if (!load.in_box(P, x, y, z, dx, dy, dz)) {
-                f = P.out_val;
-            } else {
-                ux = (x - P.ox) * P.inv_w;
-                uy = (y - P.oy) * P.inv_w;
-                uz = (z - P.oz) * P.inv_w;
-                if (ux >= 0.f && uy >= 0.f && uz >= 0.f && ux < Rf && uy < Rf && uz < Rf) {
-                    const int gx = (int)(ux * sc), gy = (int)(uy * sc), gz = (int)(uz * sc);
-                    const float4* gb = P.G + 4 * (size_t)((gz * P.Rc + gy) * P.Rc + gx);
-                    const float4 g0 = gb[0], g1 = gb[1], gl = gb[2];
-                    f = tri_in(g0, g1, __builtin_amdgcn_fractf(ux * sc), __builtin_amdgcn_fractf(uy * sc),
-                               __builtin_amdgcn_fractf(uz * sc));
-                    base = __float_as_uint(gl.x);
-                    mask = __float_as_uint(gl.y);
-                    if (lc > 0) {
-                        const int d = lc - 1;
-                        const uint32_t key = (((int)ux >> d) & 1) | ((((int)uy >> d) & 1) << 1) |
-                                             ((((int)uz >> d) & 1) << 2);
-                        push = (mask >> key) & 1u;
-                    }
-                } else {
-                    const int cx = (int)fminf(fmaxf(floorf(ux), 0.f), fr);
-                    const int cy = (int)fminf(fmaxf(floorf(uy), 0.f), fr);
-                    const int cz = (int)fminf(fmaxf(floorf(uz), 0.f), fr);
-                    f = eval_walk(P, ux, uy, uz, cx, cy, cz, morton3(cx, cy, cz));
-                }
-            }
+        uint32_t gcell = 0, mask = 0;
+        if (i < N && query_front(load, P, off + i, f, ux, uy, uz)) {
+            gcell = (((int)(uz * sc) * P.Rc + (int)(uy * sc)) * P.Rc + (int)(ux * sc));
+            const float4 g0 = P.G[2 * (size_t)gcell], g1 = P.G[2 * (size_t)gcell + 1];
+            f = tri_level(g0, g1, ux, uy, uz, sc);
+            mask = block_mask(g0, g1);
+            if (lc > 0) push = (mask >> child_key((int)ux, (int)uy, (int)uz, lc - 1)) & 1u;
         }
         const uint64_t bal = __ballot(push);
         if (push) {
@@ -402,7 +387,7 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
             Q.uz[w][pos] = uz;
             Q.f[w][pos] = f;
             Q.i[w][pos] = (uint32_t)i;
-            Q.base[w][pos] = base;
+            Q.gcell[w][pos] = gcell;
             Q.mask[w][pos] = mask;
         } else if (i < N) {
             out[off + i] = f;
@@ -411,7 +396,7 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
         if (qn >= 64) {
             lds_fence();
             const int pos = qn - 64 + lane;
-            const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.base[w][pos],
+            const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.gcell[w][pos],
                                       Q.mask[w][pos]);
             out[off + Q.i[w][pos]] = r;
             qn -= 64;
@@ -421,7 +406,7 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
     if (qn > 0) {
         lds_fence();
         if (lane < qn) {
-            const float r = fine_walk(P, Q.ux[w][lane], Q.uy[w][lane], Q.uz[w][lane], Q.f[w][lane], Q.base[w][lane],
+            const float r = fine_walk(P, Q.ux[w][lane], Q.uy[w][lane], Q.uz[w][lane], Q.f[w][lane], Q.gcell[w][lane],
                                       Q.mask[w][lane]);
             out[off + Q.i[w][lane]] = r;
         }
@@ -458,30 +443,17 @@ __global__ void k_cell_blocks(int Rc, const float* lat, float* blocks) {
     const int64_t D = Rc + 1;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         int64_t x = c % Rc, y = (c / Rc) % Rc, z = c / ((int64_t)Rc * Rc);
-        float* o = blocks + 16 * c;
+        float* o = blocks + 8 * c;
         for (int k = 0; k < 8; ++k)
             o[k] = lat[((z + ((k >> 2) & 1)) * D + (y + ((k >> 1) & 1))) * D + (x + (k & 1))];
-        o[8] = __uint_as_float(ITS_NONE);
-        o[9] = __uint_as_float(0u);
-        for (int k = 10; k < 16; ++k) o[k] = 0.f;
     }
 }
 
-// 32-byte coefficient blocks -> 64-byte node blocks with an empty child link
-__global__ void k_node_blocks(int64_t n, const float4* blocks, float4* nodes) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    nodes[4 * i] = blocks[2 * i];
-    nodes[4 * i + 1] = blocks[2 * i + 1];
-    nodes[4 * i + 2] = make_float4(__uint_as_float(ITS_NONE), __uint_as_float(0u), 0.f, 0.f);
-    nodes[4 * i + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-// every marked cell of the child level links into its parent's block: the
-// smallest child index and the bit of its key (marked cells only have marked
-// parents, so the parent block exists)
+// every marked cell of the child level links into its parent: the smallest
+// child index and the bit of its key (marked cells only have marked parents,
+// so the parent entry exists)
 __global__ void k_link(int64_t n, const uint32_t* codes, const RankWord* parent_cells, int parent_dense_R,
-                       float4* parent_blocks, int* err) {
+                       uint32_t* parent_base, uint32_t* parent_mask, int* err) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t c = codes[i], p = c >> 3;
@@ -494,9 +466,16 @@ __global__ void k_link(int64_t n, const uint32_t* codes, const RankWord* parent_
         slot = bit_find(parent_cells, p);
         if (slot == ITS_NONE) { atomicOr(err, 1); return; }
     }
-    unsigned int* link = reinterpret_cast<unsigned int*>(parent_blocks + 4 * (int64_t)slot + 2);
-    atomicMin(&link[0], (unsigned int)i);
-    atomicOr(&link[1], 1u << (c & 7));
+    atomicMin(&parent_base[slot], (uint32_t)i);
+    atomicOr(&parent_mask[slot], 1u << (c & 7));
+}
+
+// child mask bit k into the lowest mantissa bit of corner value k
+__global__ void k_embed_mask(int64_t n, const uint32_t* mask, uint32_t* blocks) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t m = mask[i];
+    for (int k = 0; k < 8; ++k) blocks[8 * i + k] = (blocks[8 * i + k] & ~1u) | ((m >> k) & 1u);
 }
 
 __global__ void k_emit_codes(const RankWord* g, int64_t nwords, uint32_t* out) {
@@ -523,7 +502,8 @@ EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_v
         P.blocks[d] = s->lv[d].blocks.p;
     }
     P.G = s->dense_blocks.p;
-    for (int d = 0; d < s->depth; ++d) P.nodes[d] = s->lv[d].nodes.p;
+    P.Gbase = s->dense_base.p;
+    for (int d = 0; d < s->depth; ++d) P.cbase[d] = s->lv[d].cbase.p;
     P.L = s->depth;
     P.R = s->R;
     P.lc = s->dense_level;
@@ -622,21 +602,29 @@ void field_build(its_svo s) {
         dense_cascade(s, lc, lat);
         const int Rc = s->lv[lc].R;
         const int64_t nc = (int64_t)Rc * Rc * Rc;
-        s->dense_blocks.ensure(4 * nc);
+        s->dense_blocks.ensure(2 * nc);
         hipLaunchKernelGGL(k_cell_blocks, dim3((unsigned)std::min<int64_t>(blocks_for(nc, kT), 65536)), dim3(kT), 0, st,
                            Rc, lat.p, (float*)s->dense_blocks.p);
         ITS_LAUNCH_CHECK();
+        s->dense_base.ensure(nc);
+        ITS_HIP(hipMemsetAsync(s->dense_base.p, 0xFF, nc * sizeof(uint32_t), st));   // ITS_NONE
     }
-    // node blocks with child links for the sparse levels below the dense one
-    DBuf<int> err(1);
-    ITS_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    // child links of every level above the finest, down to the collapsed one:
+    // first-child index per cell, and the child mask folded into its block
+    const int64_t nc_dense = (int64_t)s->lv[lc].R * s->lv[lc].R * s->lv[lc].R;
+    DBuf<uint32_t> dense_mask(nc_dense);
+    ITS_HIP(hipMemsetAsync(dense_mask.p, 0, nc_dense * sizeof(uint32_t), st));
+    std::vector<DBuf<uint32_t>> masks(lc);
     for (int d = 1; d < lc; ++d) {
         Level& L = s->lv[d];
-        L.nodes.ensure(std::max<int64_t>(4 * L.n_cells, 4));
-        hipLaunchKernelGGL(k_node_blocks, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, L.blocks.p,
-                           L.nodes.p);
-        ITS_LAUNCH_CHECK();
+        const int64_t m = std::max<int64_t>(L.n_cells, 1);
+        L.cbase.ensure(m);
+        ITS_HIP(hipMemsetAsync(L.cbase.p, 0xFF, m * sizeof(uint32_t), st));
+        masks[d].ensure(m);
+        ITS_HIP(hipMemsetAsync(masks[d].p, 0, m * sizeof(uint32_t), st));
     }
+    DBuf<int> err(1);
+    ITS_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
     for (int d = 0; d < lc; ++d) {   // link the marked cells of level d into level d+1
         Level& L = s->lv[d];
         const int64_t nwords = std::max<int64_t>(1, ((int64_t)L.R * L.R * L.R + 31) / 32);
@@ -646,11 +634,25 @@ void field_build(its_svo s) {
         const bool dense_parent = (d + 1 == lc);
         hipLaunchKernelGGL(k_link, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, codes.p,
                            s->lv[d + 1].cells.p, dense_parent ? s->lv[lc].R : 0,
-                           dense_parent ? s->dense_blocks.p : s->lv[d + 1].nodes.p, err.p);
+                           dense_parent ? s->dense_base.p : s->lv[d + 1].cbase.p,
+                           dense_parent ? dense_mask.p : masks[d + 1].p, err.p);
         ITS_LAUNCH_CHECK();
         ITS_HIP(hipStreamSynchronize(st));
     }
     ITS_REQUIRE(read1(err.p, st) == 0, ITS_E_STATE, "marked cell without a marked parent");
+    if (lc > 0) {
+        hipLaunchKernelGGL(k_embed_mask, dim3(blocks_for(nc_dense, kT)), dim3(kT), 0, st, nc_dense, dense_mask.p,
+                           (uint32_t*)s->dense_blocks.p);
+        ITS_LAUNCH_CHECK();
+    }
+    for (int d = 1; d < lc; ++d) {
+        Level& L = s->lv[d];
+        if (L.n_cells == 0) continue;
+        hipLaunchKernelGGL(k_embed_mask, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, masks[d].p,
+                           (uint32_t*)L.blocks.p);
+        ITS_LAUNCH_CHECK();
+    }
+    ITS_HIP(hipStreamSynchronize(st));
     s->field_ready = true;
 }
 

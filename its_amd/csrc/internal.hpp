@@ -20,8 +20,10 @@ struct Level {
     DBuf<RankWord> lattice; // corner lattice (R+1)^3 bits, index (x*(R+1)+y)*(R+1)+z
     // evaluation field: cells whose corners include a vertex
     DBuf<RankWord> cells;   // R^3 bits by Morton code
-    DBuf<float4> blocks;    // 2 float4 (8 corner coefficients) per marked cell
-    DBuf<float4> nodes;     // 4 float4 per marked cell: coefficients + child link
+    DBuf<float4> blocks;    // 2 float4 (8 corner coefficients) per marked cell; on the
+                            // levels linked below the collapsed one, the cell's child mask
+                            // rides in the lowest mantissa bit of each coefficient
+    DBuf<uint32_t> cbase;   // index of each marked cell's first marked child (linked levels)
     int64_t n_cells = 0;
 };
 
@@ -68,6 +70,7 @@ struct its_svo_s {
     bool has_lambda = false;
     bool field_ready = false;
     its::DBuf<float4> dense_blocks;   // per dense_level cell: corner values of levels >= dense_level
+    its::DBuf<uint32_t> dense_base;   // per dense_level cell: first marked child
     int dense_level = 0;
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};
