@@ -123,6 +123,9 @@ int its_lscg_solve(its_ctx ctx, its_svo svo, const double* b, double* lambda_ino
 /* Matrix-free products with the collocation matrix A_ij = B_j(v_i)
  * (the assembled matrix of ThinShells.cpp:556-605), binary64, host vectors. */
 int its_collocation_mul(its_ctx ctx, its_svo svo, const double* x, double* y, int transpose);
+/* The solver's Jacobi diagonal 1/||A_:j||^2 (1 for an empty column), the
+ * LeastSquareDiagonalPreconditioner of the Eigen solve; host n-vector. */
+int its_lscg_diag(its_ctx ctx, its_svo svo, double* invd);
 
 /* ---- coefficient field used by evaluation and marching cubes ---- */
 int its_svo_set_lambda(its_ctx ctx, its_svo svo, const double* lambda);
@@ -168,6 +171,49 @@ int its_mc_extract(its_ctx ctx, its_svo svo, const double origin[3], const doubl
 /* lattice values f at the (res+1)^3 grid points, x fastest (for checking) */
 int its_mc_lattice(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
                    float* values);
+/* binary64 values of the field on the octree's own level-0 lattice, (R+1)^3
+ * points over the grid box, x fastest: the lattice of mcVisualization at the
+ * octree resolution, without rounding to binary32 (for precision reports). */
+int its_lattice_f64(its_ctx ctx, its_svo svo, double* values);
+
+/* ---- learned warm start (no reference counterpart: the reference starts
+ *      its solve from zero, ThinShells.cpp:615-617) ----
+ * A network of 4 dense layers (16 -> 64 -> 64 -> 64 -> 1, GELU) predicts, for
+ * every vertex, a row-space coefficient w_i of the fit, and the solve starts
+ * from lambda0 = D A^T w with D = diag(1/||A_:j||^2).  Jacobi-preconditioned
+ * CGLS keeps its iterates in lambda0 + range(D A^T), so from such a start it
+ * converges to the same lambda as from zero: the prediction changes the
+ * iteration count, never the fitted field.
+ * Parameters (binary32 master copy, ITS_MLP_NPARAM values): W1[16][64], b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], W4[64], b4; W[k][n] maps input k to
+ * output n.  BF16 MFMA operands, binary32 accumulation. */
+#define ITS_MLP_IN 16
+#define ITS_MLP_HIDDEN 64
+#define ITS_MLP_NPARAM 9473
+typedef struct its_mlp_s* its_mlp;
+int its_mlp_create(its_ctx ctx, uint64_t seed, its_mlp* out);   /* Glorot-uniform weights, zero biases */
+int its_mlp_destroy(its_mlp net);
+int its_mlp_set_params(its_mlp net, const float* params);
+int its_mlp_get_params(its_mlp net, float* params);
+/* d_x: rows x 16 bfloat16 (raw 16-bit patterns), d_y: rows outputs */
+int its_mlp_forward_device(its_mlp net, const uint16_t* d_x, int64_t rows, float* d_y, void* stream);
+/* gradient of mean((y - t)^2) over the rows: grad (host, ITS_MLP_NPARAM), loss = the mean */
+int its_mlp_grad_device(its_mlp net, const uint16_t* d_x, const float* d_t, int64_t rows, float* grad, double* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient; loss before the step */
+int its_mlp_train_step(its_mlp net, const uint16_t* d_x, const float* d_t, int64_t rows, float lr, double* loss);
+/* features of every vertex (n x 16 bfloat16) and the cell width of its level;
+ * needs its_sdf_at_vertices */
+int its_warm_features(its_ctx ctx, its_svo svo, uint16_t* d_x, float* d_scale, void* stream);
+/* dual coefficients w of a solution lambda (lambda = D A^T w), equal shares
+ * within each group of coincident vertices; host n-vectors */
+int its_warm_dual(its_ctx ctx, its_svo svo, const double* lambda, double* w);
+/* lambda0 = D A^T w; host n-vectors */
+int its_warm_apply(its_ctx ctx, its_svo svo, const double* w, double* lambda0);
+/* features -> network -> lambda0 -> the unmodified LSCG of its_lscg_solve
+ * (b: NULL = device RHS).  seconds_predict: device time of the prediction;
+ * stats->seconds: the solve alone. */
+int its_lscg_solve_mlp(its_ctx ctx, its_svo svo, its_mlp net, const double* b, double* lambda, double tol,
+                       int64_t max_iter, its_lscg_stats* stats, double* seconds_predict);
 
 /* ---- tunables (process-wide; set by the autotuner, its_amd/autotune.py) ----
  * ITS_TUNE_DENSE_CELLS: the evaluation field collapses every level whose

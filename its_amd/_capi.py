@@ -112,6 +112,20 @@ SIGNATURES = {
     "its_tune_get": [C.c_int, C.POINTER(C.c_int64)],
     "its_timer_start": [vp],
     "its_timer_stop": [vp, C.POINTER(C.c_double)],
+    "its_lscg_diag": [vp, vp, _f64p],
+    "its_lattice_f64": [vp, vp, _f64p],
+    "its_mlp_create": [vp, C.c_uint64, C.POINTER(vp)],
+    "its_mlp_destroy": [vp],
+    "its_mlp_set_params": [vp, _f32p],
+    "its_mlp_get_params": [vp, _f32p],
+    "its_mlp_forward_device": [vp, vp, C.c_int64, vp, vp],
+    "its_mlp_grad_device": [vp, vp, vp, C.c_int64, _f32p, C.POINTER(C.c_double)],
+    "its_mlp_train_step": [vp, vp, vp, C.c_int64, C.c_float, C.POINTER(C.c_double)],
+    "its_warm_features": [vp, vp, vp, vp, vp],
+    "its_warm_dual": [vp, vp, _f64p, _f64p],
+    "its_warm_apply": [vp, vp, _f64p, _f64p],
+    "its_lscg_solve_mlp": [vp, vp, vp, vp, _f64p, C.c_double, C.c_int64, C.POINTER(LscgStats),
+                           C.POINTER(C.c_double)],
 }
 _RET = {"its_last_error": C.c_char_p, "its_version": C.c_char_p}
 
@@ -331,6 +345,43 @@ class Octree:
         del bp
         return lam, st.as_dict()
 
+    def lscg_mlp(self, net, b=None, tol=1e-6, max_iter=0):
+        """Warm-started solve: features -> network -> lambda0 = D A^T w -> LSCG."""
+        lam = np.zeros(self.num_verts)
+        st = LscgStats()
+        tp = C.c_double(0)
+        bb = _f64(b) if b is not None else None
+        check(lib().its_lscg_solve_mlp(self.ctx.handle, self._h, net.handle,
+                                       bb.ctypes.data_as(vp) if bb is not None else None, lam, tol, int(max_iter),
+                                       C.byref(st), C.byref(tp)))
+        d = st.as_dict()
+        d["seconds_predict"] = tp.value
+        return lam, d
+
+    def lscg_diag(self):
+        d = np.zeros(self.num_verts)
+        check(lib().its_lscg_diag(self.ctx.handle, self._h, d))
+        return d
+
+    def lattice_f64(self):
+        D = self.info.R + 1
+        out = np.zeros(D * D * D)
+        check(lib().its_lattice_f64(self.ctx.handle, self._h, out))
+        return out
+
+    def warm_features(self, d_x, d_scale, stream=None):
+        check(lib().its_warm_features(self.ctx.handle, self._h, vp(d_x), vp(d_scale), vp(stream) if stream else None))
+
+    def warm_dual(self, lam):
+        w = np.zeros(self.num_verts)
+        check(lib().its_warm_dual(self.ctx.handle, self._h, _f64(lam), w))
+        return w
+
+    def warm_apply(self, w):
+        lam0 = np.zeros(self.num_verts)
+        check(lib().its_warm_apply(self.ctx.handle, self._h, _f64(w), lam0))
+        return lam0
+
     def mul(self, x, transpose=False):
         y = np.zeros(self.num_verts)
         check(lib().its_collocation_mul(self.ctx.handle, self._h, _f64(x), y, 1 if transpose else 0))
@@ -377,6 +428,56 @@ class Octree:
         out = np.zeros(int(np.prod(res + 1)), np.float32)
         check(lib().its_mc_lattice(self.ctx.handle, self._h, _f64(origin), _f64(width), res, out))
         return out
+
+
+MLP_IN, MLP_HIDDEN, MLP_NPARAM = 16, 64, 9473
+
+
+class Mlp:
+    """Warm-start network (BF16 MFMA kernels); parameters as binary32."""
+
+    def __init__(self, ctx: Context, seed=0):
+        self.ctx = ctx
+        h = vp()
+        check(lib().its_mlp_create(ctx.handle, int(seed), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().its_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        check(lib().its_mlp_get_params(self._h, p))
+        return p
+
+    def set_params(self, p):
+        check(lib().its_mlp_set_params(self._h, np.ascontiguousarray(p, np.float32)))
+
+    def forward_device(self, d_x, rows, d_y, stream=None):
+        check(lib().its_mlp_forward_device(self._h, vp(d_x), int(rows), vp(d_y), vp(stream) if stream else None))
+
+    def grad_device(self, d_x, d_t, rows):
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double(0)
+        check(lib().its_mlp_grad_device(self._h, vp(d_x), vp(d_t), int(rows), g, C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, d_x, d_t, rows, lr):
+        loss = C.c_double(0)
+        check(lib().its_mlp_train_step(self._h, vp(d_x), vp(d_t), int(rows), float(lr), C.byref(loss)))
+        return loss.value
 
 
 def classify(f, inner, outer):

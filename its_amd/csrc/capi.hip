@@ -313,6 +313,9 @@ int its_sdf_at_vertices(its_ctx ctx, its_svo s, const double* V, int64_t nV, con
     s->rhs.ensure(std::max<int64_t>(n, 1));
     s->rhs_tri.ensure(std::max<int64_t>(n, 1));
     signed_distance(st, dV.p, nV, dF.p, nF, s->vxyz.p, n, s->rhs.p, s->rhs_tri.p);
+    s->rhs_nrm.ensure(std::max<int64_t>(n, 1));
+    nearest_normals(st, dV.p, dF.p, s->rhs_tri.p, n, s->rhs_nrm.p);
+    ITS_HIP(hipStreamSynchronize(st));
     s->has_rhs = true;
     if (b && n) {
         ITS_HIP(hipMemcpyAsync(b, s->rhs.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -492,6 +495,187 @@ int its_mc_lattice(its_ctx ctx, its_svo s, const double origin[3], const double 
     ITS_REQUIRE(ctx == s->ctx && origin && width && res && values, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     mc_lattice(s, origin, width, res, values);
+    ITS_API_END
+}
+
+int its_lscg_diag(its_ctx ctx, its_svo s, double* invd) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && invd, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int64_t n = s->num_verts;
+    if (n == 0) return ITS_OK;
+    DBuf<double> d(n);
+    lscg_diag(s, d.p, st);
+    ITS_HIP(hipMemcpyAsync(invd, d.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+int its_lattice_f64(its_ctx ctx, its_svo s, double* values) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && values, ITS_E_INVALID, "bad arguments");
+    ITS_REQUIRE(s->has_lambda && s->depth > 0, ITS_E_STATE, "no coefficients");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    DBuf<double> lat;
+    dense_cascade64(s, 0, lat);
+    const int64_t D = s->R + 1;
+    ITS_HIP(hipMemcpyAsync(values, lat.p, D * D * D * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+// ---- learned warm start
+static void check_mlp(its_mlp net) { ITS_REQUIRE(net, ITS_E_INVALID, "null network"); }
+
+int its_mlp_create(its_ctx ctx, uint64_t seed, its_mlp* out) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx && out, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    its_mlp net = mlp_new();
+    try {
+        mlp_init(mlp_of(net), ctx, seed);
+    } catch (...) {
+        mlp_delete(net);
+        throw;
+    }
+    *out = net;
+    ITS_API_END
+}
+int its_mlp_destroy(its_mlp net) {
+    ITS_API_BEGIN
+    if (net) {
+        DeviceGuard g(mlp_ctx(net)->device);
+        mlp_delete(net);
+    }
+    ITS_API_END
+}
+int its_mlp_set_params(its_mlp net, const float* params) {
+    ITS_API_BEGIN
+    check_mlp(net);
+    ITS_REQUIRE(params, ITS_E_INVALID, "null parameters");
+    its_ctx ctx = mlp_ctx(net);
+    DeviceGuard g(ctx->device);
+    mlp_set(net, params, ctx->stream);
+    ITS_API_END
+}
+int its_mlp_get_params(its_mlp net, float* params) {
+    ITS_API_BEGIN
+    check_mlp(net);
+    ITS_REQUIRE(params, ITS_E_INVALID, "null parameters");
+    its_ctx ctx = mlp_ctx(net);
+    DeviceGuard g(ctx->device);
+    mlp_get(net, params, ctx->stream);
+    ITS_API_END
+}
+int its_mlp_forward_device(its_mlp net, const uint16_t* d_x, int64_t rows, float* d_y, void* stream) {
+    ITS_API_BEGIN
+    check_mlp(net);
+    ITS_REQUIRE(rows >= 0 && (rows == 0 || (d_x && d_y)), ITS_E_INVALID, "bad arguments");
+    its_ctx ctx = mlp_ctx(net);
+    DeviceGuard g(ctx->device);
+    mlp_forward(mlp_of(net), d_x, rows, d_y, stream ? (hipStream_t)stream : ctx->stream);
+    ITS_API_END
+}
+int its_mlp_grad_device(its_mlp net, const uint16_t* d_x, const float* d_t, int64_t rows, float* grad, double* loss) {
+    ITS_API_BEGIN
+    check_mlp(net);
+    ITS_REQUIRE(rows > 0 && d_x && d_t, ITS_E_INVALID, "bad arguments");
+    its_ctx ctx = mlp_ctx(net);
+    DeviceGuard g(ctx->device);
+    mlp_grad(mlp_of(net), d_x, d_t, rows, ctx->stream);
+    mlp_read_grad(net, grad, loss, rows, ctx->stream);
+    ITS_API_END
+}
+int its_mlp_train_step(its_mlp net, const uint16_t* d_x, const float* d_t, int64_t rows, float lr, double* loss) {
+    ITS_API_BEGIN
+    check_mlp(net);
+    ITS_REQUIRE(rows > 0 && d_x && d_t && lr > 0, ITS_E_INVALID, "bad arguments");
+    its_ctx ctx = mlp_ctx(net);
+    DeviceGuard g(ctx->device);
+    mlp_grad(mlp_of(net), d_x, d_t, rows, ctx->stream);
+    mlp_adam(mlp_of(net), lr, ctx->stream);
+    if (loss) mlp_read_grad(net, nullptr, loss, rows, ctx->stream);
+    ITS_API_END
+}
+int its_warm_features(its_ctx ctx, its_svo s, uint16_t* d_x, float* d_scale, void* stream) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && d_x && d_scale, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    warm_features(s, d_x, d_scale, stream ? (hipStream_t)stream : ctx->stream);
+    ITS_API_END
+}
+int its_warm_dual(its_ctx ctx, its_svo s, const double* lambda, double* w) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && lambda && w, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int64_t n = s->num_verts;
+    if (n == 0) return ITS_OK;
+    DBuf<double> dl = upload(lambda, n, st);
+    DBuf<double> dw(n);
+    warm_dual(s, dl.p, dw.p);
+    ITS_HIP(hipMemcpyAsync(w, dw.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+int its_warm_apply(its_ctx ctx, its_svo s, const double* w, double* lambda0) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && w && lambda0, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int64_t n = s->num_verts;
+    if (n == 0) return ITS_OK;
+    DBuf<double> dw = upload(w, n, st);
+    DBuf<double> dl(n);
+    warm_apply(s, dw.p, dl.p, st);
+    ITS_HIP(hipMemcpyAsync(lambda0, dl.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, double* lambda, double tol,
+                       int64_t max_iter, its_lscg_stats* stats, double* seconds_predict) {
+    ITS_API_BEGIN
+    check_svo(s);
+    check_mlp(net);
+    ITS_REQUIRE(ctx == s->ctx && mlp_ctx(net) == ctx && lambda, ITS_E_INVALID, "bad arguments");
+    ITS_REQUIRE(tol > 0, ITS_E_INVALID, "tolerance must be positive");
+    ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "the features need its_sdf_at_vertices");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int64_t n = s->num_verts;
+    ITS_REQUIRE(n > 0, ITS_E_STATE, "empty octree");
+    DBuf<double> db;
+    const double* d_b = s->rhs.p;
+    if (b) {
+        db = upload(b, n, st);
+        d_b = db.p;
+    }
+    ITS_HIP(hipEventRecord(ctx->ev0, st));
+    s->lambda64.ensure(n);
+    DBuf<uint16_t> x(16 * n);
+    DBuf<float> scale(n), y(n);
+    DBuf<double> w(n);
+    warm_features(s, x.p, scale.p, st);
+    mlp_forward(mlp_of(net), x.p, n, y.p, st);
+    scale_to_f64(n, y.p, scale.p, w.p, st);
+    warm_apply(s, w.p, s->lambda64.p, st);
+    ITS_HIP(hipEventRecord(ctx->ev1, st));
+    ITS_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    ITS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (seconds_predict) *seconds_predict = ms * 1e-3;
+    lscg_solve(s, d_b, s->lambda64.p, tol, max_iter, true, stats);
+    ITS_HIP(hipMemcpyAsync(lambda, s->lambda64.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    s->has_lambda = true;
+    field_build(s);
     ITS_API_END
 }
 

@@ -658,25 +658,33 @@ void field_build(its_svo s) {
 
 // f at every lattice point of level `stop` (values of levels >= stop only
 // when stop > 0), binary32, x fastest
-void dense_cascade(its_svo s, int stop, DBuf<float>& out) {
+void dense_cascade64(its_svo s, int stop, DBuf<double>& out) {
     hipStream_t st = s->ctx->stream;
-    DBuf<double> prev, cur;
+    DBuf<double> cur;
     int Dprev = 0;
     for (int d = s->depth - 1; d >= stop; --d) {
         const int D = s->lv[d].R + 1;
         const int64_t n = (int64_t)D * D * D;
         cur.alloc(n);
         unsigned nb = (unsigned)std::min<int64_t>(blocks_for(n, kT), 65536);
-        hipLaunchKernelGGL(k_dense_level, dim3(nb), dim3(kT), 0, st, D, d == s->depth - 1 ? nullptr : prev.p, Dprev,
+        hipLaunchKernelGGL(k_dense_level, dim3(nb), dim3(kT), 0, st, D, d == s->depth - 1 ? nullptr : out.p, Dprev,
                            s->lv[d].lattice.p, s->esum_verts[d], s->lambda64.p, cur.p);
         ITS_LAUNCH_CHECK();
-        prev = std::move(cur);
+        out = std::move(cur);
         Dprev = D;
     }
-    const int64_t n = (int64_t)Dprev * Dprev * Dprev;
+    ITS_HIP(hipStreamSynchronize(st));
+}
+
+void dense_cascade(its_svo s, int stop, DBuf<float>& out) {
+    hipStream_t st = s->ctx->stream;
+    DBuf<double> lat;
+    dense_cascade64(s, stop, lat);
+    const int D = s->lv[stop].R + 1;
+    const int64_t n = (int64_t)D * D * D;
     out.ensure(n);
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<int64_t>(blocks_for(n, kT), 65536)), dim3(kT), 0, st, n,
-                       prev.p, out.p);
+                       lat.p, out.p);
     ITS_LAUNCH_CHECK();
     ITS_HIP(hipStreamSynchronize(st));
 }

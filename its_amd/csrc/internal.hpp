@@ -65,6 +65,7 @@ struct its_svo_s {
     // right-hand side and coefficient field
     its::DBuf<double> rhs;
     its::DBuf<int32_t> rhs_tri;     // nearest triangle per vertex
+    its::DBuf<float4> rhs_nrm;      // unit normal of that triangle
     bool has_rhs = false;
     its::DBuf<double> lambda64;
     bool has_lambda = false;
@@ -84,6 +85,7 @@ void svo_build(its_svo s, const double* d_tris, int64_t T);
 // eval.hip
 void field_build(its_svo s);   // cells + blocks from lambda64
 void dense_cascade(its_svo s, int stop, DBuf<float>& out);
+void dense_cascade64(its_svo s, int stop, DBuf<double>& out);
 void eval_points(its_svo s, const float* x, const float* y, const float* z, int64_t N, const double box_end[3],
                  int mode, float out_val, float* f, hipStream_t st);
 void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box_end[3], int mode, float out_val,
@@ -100,4 +102,26 @@ void mc_lattice(its_svo s, const double origin[3], const double width[3], const 
 // sdf.hip
 void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_t* d_F, int64_t nF,
                      const double* d_pts, int64_t N, double* d_out, int32_t* d_tri);
+void nearest_normals(hipStream_t st, const double* d_V, const int32_t* d_F, const int32_t* d_tri, int64_t N,
+                     float4* d_nrm);
+// lscg.hip: warm start in the row space of A
+void warm_dual(its_svo s, const double* d_lam, double* d_w);
+void warm_apply(its_svo s, const double* d_w, double* d_lam0, hipStream_t st);
+void lscg_diag(its_svo s, double* d_out, hipStream_t st);
+// warm.hip
+void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st);
+// mlp.hip
+struct Mlp;
+void mlp_init(Mlp& m, its_ctx ctx, uint64_t seed);
+void mlp_forward(Mlp& m, const uint16_t* d_x, int64_t n, float* d_y, hipStream_t st);
+void mlp_grad(Mlp& m, const uint16_t* d_x, const float* d_t, int64_t n, hipStream_t st);
+void mlp_adam(Mlp& m, float lr, hipStream_t st);
+Mlp& mlp_of(its_mlp net);
+its_ctx mlp_ctx(its_mlp net);
+its_mlp mlp_new();
+void mlp_delete(its_mlp net);
+void mlp_set(its_mlp net, const float* h, hipStream_t st);
+void mlp_get(its_mlp net, float* h, hipStream_t st);
+void mlp_read_grad(its_mlp net, float* grad, double* loss, int64_t n, hipStream_t st);
+void scale_to_f64(int64_t n, const float* y, const float* s, double* w, hipStream_t st);
 }  // namespace its

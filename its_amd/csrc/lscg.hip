@@ -699,6 +699,85 @@ int64_t inner_cgls(Fit* f, const double* rhs, double rel_tol, int64_t max_it, hi
 }
 }  // namespace
 
+// ---- warm start in the row space of A ----
+//
+// Jacobi-preconditioned CGLS keeps every iterate in x0 + range(D A^T), with
+// D = diag(1/||A_:j||^2).  From x0 = 0 it converges to the one solution
+// lambda* in range(D A^T); from any x0 = D A^T w0 it converges to the same
+// lambda*, whatever w0 is.  A warm start of that form therefore changes the
+// iteration count but not the fitted field.
+//
+// The dual coefficients of a solution lambda = D A^T w: with A = Dpick G and
+// G = (I - P)^-1, A^T w = (I - P^T)^-1 Dpick^T w, so the coincidence-group
+// sums W = Dpick^T w are W = (I - P^T)(lambda / D): one restriction pass
+// without the cascade.  Only the group sums are determined; each vertex gets
+// an equal share of its group's sum (the minimum-norm w).
+namespace {
+__global__ void k_dual_u(int64_t n, const double* lam, const double* invd, double* u) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = lam[i] / invd[i];
+}
+__global__ void k_dual_restrict(int64_t n, const uint32_t* ptr, const uint32_t* ridx, const double* u, double* W) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double s = 0;
+    for (uint32_t k = ptr[c]; k < ptr[c + 1]; ++k) {
+        uint32_t v = ridx[k];
+        s += u[v & 0x0FFFFFFFu] / (double)(1u << (v >> 28));
+    }
+    W[c] = u[c] - s;
+}
+__global__ void k_group_count(int64_t n, const uint32_t* down, uint32_t* cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[down[i]], 1u);
+}
+__global__ void k_group_split(int64_t n, const uint32_t* down, const double* W, const uint32_t* cnt, double* w) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = down[i];
+    w[i] = W[j] / (double)cnt[j];
+}
+__global__ void k_scale_invd(int64_t n, const double* invd, double* x) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= invd[i];
+}
+}  // namespace
+
+void warm_dual(its_svo s, const double* d_lam, double* d_w) {
+    Fit* f = fit_get(s);
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = f->n;
+    DBuf<double> u(n), W(n);
+    DBuf<uint32_t> cnt(n);
+    ITS_HIP(hipMemsetAsync(cnt.p, 0, n * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_dual_u, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, d_lam, (const double*)f->invd64.p, u.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dual_restrict, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, (const uint32_t*)f->res_ptr.p,
+                       (const uint32_t*)f->res_idx.p, (const double*)u.p, W.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_group_count, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, (const uint32_t*)f->down.p, cnt.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_group_split, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, (const uint32_t*)f->down.p,
+                       (const double*)W.p, (const uint32_t*)cnt.p, d_w);
+    ITS_LAUNCH_CHECK();
+    ITS_HIP(hipStreamSynchronize(st));
+}
+
+// the Jacobi diagonal D = diag(1/||A_:j||^2) of the solver
+void lscg_diag(its_svo s, double* d_out, hipStream_t st) {
+    Fit* f = fit_get(s);
+    ITS_HIP(hipMemcpyAsync(d_out, f->invd64.p, f->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+}
+
+// lambda0 = D A^T w (binary64); d_lam0 must not alias d_w
+void warm_apply(its_svo s, const double* d_w, double* d_lam0, hipStream_t st) {
+    Fit* f = fit_get(s);
+    apply_AT64(f, d_w, d_lam0, st);
+    hipLaunchKernelGGL(k_scale_invd, dim3(blocks_for(f->n, kT)), dim3(kT), 0, st, f->n,
+                       (const double*)f->invd64.p, d_lam0);
+    ITS_LAUNCH_CHECK();
+}
+
 void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose) {
     Fit* f = fit_get(s);
     hipStream_t st = s->ctx->stream;

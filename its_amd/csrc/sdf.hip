@@ -135,4 +135,27 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
     ITS_HIP(hipStreamSynchronize(st));
 }
 
+namespace {
+// unit normal of each point's nearest triangle (zero for a degenerate one)
+__global__ void k_tri_normal(int64_t N, const double* V, const int32_t* F, const int32_t* tri, float4* nrm) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int32_t t = tri[i];
+    const double* a = V + 3 * (int64_t)F[3 * (int64_t)t];
+    const double* b = V + 3 * (int64_t)F[3 * (int64_t)t + 1];
+    const double* c = V + 3 * (int64_t)F[3 * (int64_t)t + 2];
+    const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2], z = u[0] * v[1] - u[1] * v[0];
+    const double l = sqrt(x * x + y * y + z * z);
+    nrm[i] = l > 0 ? make_float4((float)(x / l), (float)(y / l), (float)(z / l), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+}  // namespace
+
+void nearest_normals(hipStream_t st, const double* d_V, const int32_t* d_F, const int32_t* d_tri, int64_t N,
+                     float4* d_nrm) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_tri_normal, dim3(blocks_for(N, kT)), dim3(kT), 0, st, N, d_V, d_F, d_tri, d_nrm);
+    ITS_LAUNCH_CHECK();
+}
+
 }  // namespace its

@@ -216,6 +216,48 @@ class Matrix:
         return x, stats
 
 
+def lscg_operator(mul, tmul, invdiag, b, tol=1e-6, max_iter=None, x0=None):
+    """Eigen's LeastSquaresConjugateGradient (the algorithm of ora_lscg, as
+    called at ThinShells.cpp:615-617) in binary64 numpy over an operator:
+    mul(x) = A x, tmul(r) = A^T r, invdiag = 1/||A_:j||^2 (1 for an empty
+    column).  Returns (x, stats) like Matrix.lscg."""
+    b = np.asarray(b, np.float64)
+    n = len(b)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    max_iter = 2 * n if max_iter is None else max_iter
+    residual = b - mul(x)
+    nres = tmul(residual)
+    rhs2 = float(np.dot(tmul(b), tmul(b)))
+    it = 0
+    if rhs2 == 0:
+        x[:] = 0
+    else:
+        thr = tol * tol * rhs2
+        rn2 = float(np.dot(nres, nres))
+        if rn2 >= thr:
+            p = invdiag * nres
+            abs_new = float(np.dot(nres, p))
+            while it < max_iter:
+                t = mul(p)
+                alpha = abs_new / float(np.dot(t, t))
+                x += alpha * p
+                residual -= alpha * t
+                nres = tmul(residual)
+                rn2 = float(np.dot(nres, nres))
+                if rn2 < thr:
+                    break
+                z = invdiag * nres
+                abs_old = abs_new
+                abs_new = float(np.dot(nres, z))
+                p = z + (abs_new / abs_old) * p
+                it += 1
+    r = mul(x) - b
+    nr = tmul(r)
+    return x, {"iterations": it, "normal_rel": float(np.sqrt(np.dot(nr, nr) / rhs2)) if rhs2 > 0 else 0.0,
+               "residual_rel": float(np.linalg.norm(r) / max(np.linalg.norm(b), 1e-300)),
+               "residual_abs": float(np.linalg.norm(r))}
+
+
 def marching_cubes(svo: SVO, lam, origin, width, res, iso):
     L = lib()
     h = L.ora_marching_cubes(svo._h, _f64(lam), _f64(origin), _f64(width), np.asarray(res, np.int32), iso)

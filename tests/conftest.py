@@ -16,6 +16,11 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch carries its own HIP runtime; it has to initialise the device before
+    # the library's runtime does, or torch finds no GPU in this process
+    import torch
+
+    torch.cuda.init()
     from its_amd import _capi
 
     return _capi.Context(0)

@@ -1,0 +1,80 @@
+"""C++ host classes (its_amd/host/its_host.hpp) over the C ABI.
+
+The example its_amd/host/shell_demo.cpp is a main.cpp-style caller
+(reference src/main.cpp:165-190): ThinShells(file, R, R, R, true, 1.0),
+creatShell, multiPointQuery and MC::marching_cubes.  It is built by
+__graft_entry__.build(); these tests run it and compare its summary with the
+same steps driven through the Python binding.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEMO = os.path.join(ROOT, "its_amd", "host", "shell_demo")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "bunny.npz")
+
+
+def _write_off(path, V, F):
+    with open(path, "w") as fh:
+        fh.write(f"OFF\n{len(V)} {len(F)} 0\n")
+        for v in V:
+            fh.write("%.17g %.17g %.17g\n" % tuple(v))
+        for f in F:
+            fh.write("3 %d %d %d\n" % tuple(f))
+
+
+def _queries(origin, width, nq):
+    i = np.arange(1, nq + 1, dtype=np.float64)[:, None]
+    a = np.arange(3, dtype=np.float64)[None, :]
+    u = np.fmod(0.6180339887498949 * i * (a + 1) + 0.1 * a, 1.0)
+    return origin - 0.05 * width + 1.1 * u * width
+
+
+def test_demo_is_built():
+    assert os.path.exists(DEMO), "run __graft_entry__.build()"
+    r = subprocess.run([DEMO], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_demo_reports_missing_file(tmp_path):
+    r = subprocess.run([DEMO, str(tmp_path / "none.off"), "16"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "its error" in r.stderr
+
+
+@pytest.mark.gpu
+def test_demo_matches_python_path(ctx, tmp_path):
+    from its_amd import _capi, meshes
+    d = np.load(GOLDEN)
+    V, F = d["V"], d["F"].astype(np.int32)
+    off = tmp_path / "bunny.off"
+    _write_off(off, V, F)
+    R, nq = 64, 3000
+    print(f"[host demo] bunny R={R}", flush=True)
+    r = subprocess.run([DEMO, str(off), str(R), str(nq)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+
+    Vn = _capi.model_to_unit_cube(V)
+    origin, end, width = _capi.uniform_bbox(Vn)
+    svo = _capi.Octree(ctx, meshes.triangles(Vn, F), origin, width, R)
+    b = svo.sdf_at_vertices(Vn, F)
+    lam, st = svo.lscg(b, tol=1e-6)
+    f = svo.eval(Vn, end, _capi.EVAL_SUPPORT, 0.0)
+    inner, outer = float(f.min()), float(f.max())
+    q = _queries(origin, width, nq)
+    fq = svo.eval(q, end, _capi.EVAL_SUPPORT, np.float32(outer))
+    cls = _capi.classify(fq, inner, outer)
+    tri, mst = svo.marching_cubes(origin, width, (R, R, R), 0.0)
+    assert got["tree_depth"] == svo.depth and got["num_nodes"] == svo.num_nodes
+    assert got["num_verts"] == svo.num_verts and got["num_voxels"] == svo.num_voxels
+    assert got["lambda_sum"] == float(np.sum(lam)) or abs(got["lambda_sum"] - np.sum(lam)) <= 1e-12 * np.abs(lam).sum()
+    assert abs(got["lambda_abs"] - np.abs(lam).sum()) <= 1e-12 * np.abs(lam).sum()
+    assert np.float32(got["inner_iso"]) == np.float32(inner) and np.float32(got["outer_iso"]) == np.float32(outer)
+    assert [got["inside"], got["shell"], got["outside"]] == [int((cls == -1).sum()), int((cls == 0).sum()),
+                                                              int((cls == 1).sum())]
+    assert got["mc_verts"] == mst["num_verts"]
+    svo.close()
