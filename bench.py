@@ -10,6 +10,9 @@ A step is one evaluation of f at all 10^7 points (its_bspline_eval_device,
 support semantics of the reference CPU query path).  The JSON line also
 carries:
 - the cold-start LSCG time to a 1e-6 normal residual on the same octree;
+- marching cubes on the 256^3 grid of the octree box (iso 0);
+- the learned warm start against the cold start on held-out noisy
+  icospheres at R = 256 (weights its_amd/data/warm_mlp.npz);
 - the roofline of the evaluation kernel (algorithmic bytes / measured launch
   time);
 - the oracle CPU path, timed on this host on a bounded sample.
@@ -75,6 +78,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--warm-meshes", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the timed region, copy this many MiB device to device (a dispatch of known size "
+                         "for calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box).", file=sys.stderr)
@@ -116,6 +123,14 @@ def main():
     st = lscg_runs[-1]
     log(f"[bench] lscg cold: {t_lscg*1e3:.2f} ms, iters={st['iterations']} normal_rel={st['normal_rel']:.2e}")
 
+    # marching cubes on the 256^3 grid of the octree box
+    mc_runs = [svo.marching_cubes(origin, width, (R, R, R), 0.0)[1] for _ in range(3)]
+    mc = {"res": R, "vertices": mc_runs[-1]["num_verts"],
+          "seconds_lattice": float(np.median([m["seconds_eval"] for m in mc_runs])),
+          "seconds_extract": float(np.median([m["seconds_extract"] for m in mc_runs]))}
+    log(f"[bench] mc 256^3: {mc['vertices']} vertices, lattice {mc['seconds_lattice']*1e3:.2f} ms, "
+        f"extract {mc['seconds_extract']*1e3:.2f} ms")
+
     # queries resident in HBM
     g = torch.Generator(device=dev)
     g.manual_seed(1234)
@@ -144,6 +159,14 @@ def main():
     med_ms = float(np.median(launch_ms))
     value = N_QUERIES * args.steps / elapsed
 
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib * (1 << 18), device=dev, dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # roofline of the evaluation kernel: 16 B/query (12 B of binary32 xyz in,
     # 4 B of f out) + 8 B per unknown (one pass over the coefficient field)
     alg_bytes = 16.0 * N_QUERIES + 8.0 * svo.num_verts
@@ -153,6 +176,26 @@ def main():
     if os.path.exists(pmc):
         with open(pmc) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
+
+    warm = None
+    wfile = os.path.join(ROOT, "its_amd", "data", "warm_mlp.npz")
+    if os.path.exists(wfile) and args.warm_meshes > 0:
+        from its_amd import warm_train
+        net = _capi.Mlp(ctx)
+        net.set_params(np.load(wfile)["params"])
+        qw = np.random.default_rng(12345).uniform(0, 1, size=(20000, 3))
+        rs = [warm_train.evaluate(ctx, torch, net, 500000 + k, 256, 1e-6, qw) for k in range(args.warm_meshes)]
+        net.close()
+        tc, tw = sum(r["t_cold"] for r in rs), sum(r["t_warm"] for r in rs)
+        warm = {"workload": "held-out noisy icospheres (level 4, 5,120 triangles), R=256, tol 1e-6",
+                "meshes": len(rs), "mean_iters_cold": float(np.mean([r["iters_cold"] for r in rs])),
+                "mean_iters_warm": float(np.mean([r["iters_warm"] for r in rs])),
+                "mean_ms_cold": tc / len(rs) * 1e3, "mean_ms_warm": tw / len(rs) * 1e3,
+                "mean_ms_predict": float(np.mean([r["t_predict"] for r in rs])) * 1e3,
+                "speedup": tc / tw, "target_speedup": 3.0,
+                "max_f_diff_warm_vs_cold": float(max(r["f_maxdiff"] for r in rs))}
+        log(f"[bench] warm start: {warm['mean_iters_cold']:.1f} -> {warm['mean_iters_warm']:.1f} iterations, "
+            f"speedup {warm['speedup']:.2f}")
 
     cpu = None
     if not args.no_cpu:
@@ -184,7 +227,8 @@ def main():
         "eval_kernel_ms": {"mean": avg_ms, "median": med_ms},
         "lscg_cold": {"seconds": t_lscg, "iterations": st["iterations"], "normal_rel": st["normal_rel"],
                       "residual_rel": st["residual_rel"]},
-        "lscg_warm": None,
+        "lscg_warm": warm,
+        "mc": mc,
         "setup_seconds": {"svo_build": t_svo, "sdf_rhs": t_sdf},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

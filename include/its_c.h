@@ -215,6 +215,28 @@ int its_warm_apply(its_ctx ctx, its_svo svo, const double* w, double* lambda0);
 int its_lscg_solve_mlp(its_ctx ctx, its_svo svo, its_mlp net, const double* b, double* lambda, double tol,
                        int64_t max_iter, its_lscg_stats* stats, double* seconds_predict);
 
+/* ---- many meshes per launch (no reference counterpart: the reference fits
+ *      one model per process) ----
+ * A batch holds octrees of one context, each with its right-hand side
+ * (its_sdf_at_vertices).  its_batch_fit solves all of them in one persistent
+ * launch (one workgroup per mesh runs that mesh's whole mixed-precision LSCG,
+ * the algorithm and stopping rule of its_lscg_solve), captured once as a
+ * hipGraph and replayed on later calls; stats: one per mesh, seconds: the
+ * launch.  warm != 0 starts every mesh from its current coefficients.  The
+ * coefficients stay on the octrees; their evaluation fields are rebuilt by
+ * its_batch_fields.  its_batch_eval_device evaluates f for all meshes in one
+ * launch: queries [offsets[m], offsets[m+1]) of the device SoA arrays belong
+ * to mesh m (offsets: host, count + 1 values), support semantics inside each
+ * octree's box, out_val outside. */
+typedef struct its_batch_s* its_batch;
+int its_batch_create(its_ctx ctx, its_batch* out);
+int its_batch_destroy(its_batch b);
+int its_batch_add(its_batch b, its_svo svo);
+int its_batch_fit(its_batch b, double tol, int64_t max_iter, int warm, its_lscg_stats* stats, double* seconds);
+int its_batch_fields(its_batch b);
+int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const float* d_z, const int64_t* offsets,
+                          float out_val, float* d_f, double* seconds);
+
 /* ---- tunables (process-wide; set by the autotuner, its_amd/autotune.py) ----
  * ITS_TUNE_DENSE_CELLS: the evaluation field collapses every level whose
  *   cell count is at most this into one dense block grid (default 32768).

@@ -114,6 +114,13 @@ SIGNATURES = {
     "its_timer_stop": [vp, C.POINTER(C.c_double)],
     "its_lscg_diag": [vp, vp, _f64p],
     "its_lattice_f64": [vp, vp, _f64p],
+    "its_batch_create": [vp, C.POINTER(vp)],
+    "its_batch_destroy": [vp],
+    "its_batch_add": [vp, vp],
+    "its_batch_fit": [vp, C.c_double, C.c_int64, C.c_int, vp, C.POINTER(C.c_double)],
+    "its_batch_fields": [vp],
+    "its_batch_eval_device": [vp, vp, vp, vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_float,
+                              vp, C.POINTER(C.c_double)],
     "its_mlp_create": [vp, C.c_uint64, C.POINTER(vp)],
     "its_mlp_destroy": [vp],
     "its_mlp_set_params": [vp, _f32p],
@@ -428,6 +435,48 @@ class Octree:
         out = np.zeros(int(np.prod(res + 1)), np.float32)
         check(lib().its_mc_lattice(self.ctx.handle, self._h, _f64(origin), _f64(width), res, out))
         return out
+
+
+class Batch:
+    """Octrees fitted and evaluated together (one launch each)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = vp()
+        check(lib().its_batch_create(ctx.handle, C.byref(h)))
+        self._h = h
+        self.octrees = []
+
+    def add(self, svo):
+        check(lib().its_batch_add(self._h, svo.handle))
+        self.octrees.append(svo)
+
+    def fit(self, tol=1e-6, max_iter=0, warm=False):
+        n = len(self.octrees)
+        st = (LscgStats * n)()
+        sec = C.c_double(0)
+        check(lib().its_batch_fit(self._h, tol, int(max_iter), 1 if warm else 0, C.cast(st, vp), C.byref(sec)))
+        return [s.as_dict() for s in st], sec.value
+
+    def fields(self):
+        check(lib().its_batch_fields(self._h))
+
+    def eval_device(self, dx, dy, dz, offsets, d_out, out_val=0.0):
+        sec = C.c_double(0)
+        check(lib().its_batch_eval_device(self._h, vp(dx), vp(dy), vp(dz), np.ascontiguousarray(offsets, np.int64),
+                                          float(out_val), vp(d_out), C.byref(sec)))
+        return sec.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().its_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 MLP_IN, MLP_HIDDEN, MLP_NPARAM = 16, 64, 9473

@@ -1,0 +1,375 @@
+// Many meshes in one launch (gfx950).
+//
+// No reference counterpart: the reference fits one model per process.
+//
+// Fit: one persistent kernel, one 1024-thread workgroup per mesh, runs each
+// mesh's whole solve -- the binary64 refinement loop and the binary32 CGLS
+// inside it, the same algorithm, stopping rule and operation order as
+// its_lscg_solve -- with the level passes of the matrix-free operator
+// separated by workgroup barriers instead of kernel boundaries.  No
+// grid-wide synchronisation is needed, so every workgroup runs to its own
+// end; the launch is captured once per batch as a hipGraph and replayed.
+// Every scalar of the recurrence is computed by all threads from the same
+// block sums, so all threads take the same branches and reach every barrier.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "fit.hpp"
+
+namespace its {
+namespace {
+constexpr int kB = 1024;
+constexpr int kBW = kB / 64;
+
+struct FitDesc {
+    int64_t n;
+    int L;
+    int warm;
+    int64_t max_iter;
+    double tol;
+    int64_t off[kMaxLevels + 1];
+    const uint32_t *pro_ptr, *pro_idx, *res_ptr, *res_idx, *down, *up;
+    const uint8_t* finest;
+    const float* invd32;
+    const double* b;
+    double* x64;
+    float *x, *r, *p, *z, *t, *g, *h;
+    double *g64, *r64, *h64;
+    double* stats;   // 8 values, see its_batch_fit
+};
+
+// block sum, result in every thread (fixed order: waves, then wave 0..15)
+__device__ inline double bsum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0;
+    for (int k = 0; k < kBW; ++k) s += sh[k];
+    return s;
+}
+
+// g = G x: top-down prolongation, binary64
+__device__ void prolong64(const FitDesc& D, const double* x, double* g) {
+    for (int d = D.L - 1; d >= 0; --d) {
+        for (int64_t i = D.off[d] + threadIdx.x; i < D.off[d + 1]; i += kB) {
+            const uint32_t k0 = D.pro_ptr[i], k1 = D.pro_ptr[i + 1];
+            double s = 0;
+            for (uint32_t k = k0; k < k1; ++k) s += g[D.pro_idx[k]];
+            const double w = (k1 > k0) ? 1.0 / (double)(k1 - k0) : 0.0;
+            g[i] = x[i] + w * s;
+        }
+        __syncthreads();
+    }
+}
+// h = A^T r: sums along the coincidence chains, then bottom-up restriction
+template <class T>
+__device__ void apply_AT(const FitDesc& D, const T* r, T* h) {
+    for (int64_t i = threadIdx.x; i < D.n; i += kB) {
+        if (!D.finest[i]) {
+            h[i] = 0;
+            continue;
+        }
+        T s = 0;
+        for (uint32_t j = (uint32_t)i; j != ITS_NONE; j = D.up[j]) s += r[j];
+        h[i] = s;
+    }
+    __syncthreads();
+    for (int d = 1; d < D.L; ++d) {
+        for (int64_t c = D.off[d] + threadIdx.x; c < D.off[d + 1]; c += kB) {
+            T s = 0;
+            for (uint32_t k = D.res_ptr[c]; k < D.res_ptr[c + 1]; ++k) {
+                const uint32_t v = D.res_idx[k];
+                s += ((T)1 / (T)(1u << (v >> 28))) * h[v & 0x0FFFFFFFu];
+            }
+            h[c] += s;
+        }
+        __syncthreads();
+    }
+}
+// r64 = b - A x64; returns ||r64||^2
+__device__ double residual64(const FitDesc& D, double* sh) {
+    prolong64(D, D.x64, D.g64);
+    double acc = 0;
+    for (int64_t i = threadIdx.x; i < D.n; i += kB) {
+        const double v = D.b[i] - D.g64[D.down[i]];
+        D.r64[i] = v;
+        acc += v * v;
+    }
+    __syncthreads();
+    return bsum(acc, sh);
+}
+__device__ double dot_self(const double* a, int64_t n, double* sh) {
+    double acc = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kB) acc += a[i] * a[i];
+    return bsum(acc, sh);
+}
+
+// binary32 CGLS on A dx = r64 to a relative normal residual rel_tol;
+// returns the iterations (the recurrence of the single-mesh solver)
+__device__ int64_t inner_cgls(const FitDesc& D, double rel_tol, int64_t max_it, double* sh) {
+    const int64_t n = D.n;
+    for (int64_t i = threadIdx.x; i < n; i += kB) {
+        D.x[i] = 0.f;
+        D.p[i] = 0.f;
+        D.r[i] = (float)D.r64[i];
+    }
+    __syncthreads();
+    apply_AT<float>(D, D.r, D.h);
+    double a = 0, c = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kB) {
+        const float v = D.h[i], zi = D.invd32[i] * v;
+        D.z[i] = zi;
+        a += (double)v * (double)v;
+        c += (double)v * (double)zi;
+    }
+    __syncthreads();
+    const double rn2_0 = bsum(a, sh);
+    double abs_new = bsum(c, sh);
+    if (abs_new == 0.0) return 0;
+    const double thresh = rel_tol * rel_tol * rn2_0;
+    double beta = 0.0;
+    int64_t it = 0;
+    while (it < max_it) {
+        // p = z + beta p; g = G p
+        for (int d = D.L - 1; d >= 0; --d) {
+            for (int64_t i = D.off[d] + threadIdx.x; i < D.off[d + 1]; i += kB) {
+                const float pi = D.z[i] + (float)beta * D.p[i];
+                D.p[i] = pi;
+                const uint32_t k0 = D.pro_ptr[i], k1 = D.pro_ptr[i + 1];
+                float s = 0;
+                for (uint32_t k = k0; k < k1; ++k) s += D.g[D.pro_idx[k]];
+                const float w = (k1 > k0) ? 1.f / (float)(k1 - k0) : 0.f;
+                D.g[i] = pi + w * s;
+            }
+            __syncthreads();
+        }
+        // t = D g, alpha
+        double tt = 0;
+        for (int64_t i = threadIdx.x; i < n; i += kB) {
+            const float v = D.g[D.down[i]];
+            D.t[i] = v;
+            tt += (double)v * (double)v;
+        }
+        __syncthreads();
+        tt = bsum(tt, sh);
+        const float alpha = (float)(tt > 0 ? abs_new / tt : 0.0);
+        // x += alpha p; r -= alpha t along each chain; h = D^T r
+        for (int64_t i = threadIdx.x; i < n; i += kB) {
+            D.x[i] = fmaf(alpha, D.p[i], D.x[i]);
+            if (!D.finest[i]) {
+                D.h[i] = 0.f;
+                continue;
+            }
+            float s = 0.f;
+            for (uint32_t j = (uint32_t)i; j != ITS_NONE; j = D.up[j]) {
+                const float rj = fmaf(-alpha, D.t[j], D.r[j]);
+                D.r[j] = rj;
+                s += rj;
+            }
+            D.h[i] = s;
+        }
+        __syncthreads();
+        for (int d = 1; d < D.L; ++d) {
+            for (int64_t cc = D.off[d] + threadIdx.x; cc < D.off[d + 1]; cc += kB) {
+                float s = 0;
+                for (uint32_t k = D.res_ptr[cc]; k < D.res_ptr[cc + 1]; ++k) {
+                    const uint32_t v = D.res_idx[k];
+                    s += (1.f / (float)(1u << (v >> 28))) * D.h[v & 0x0FFFFFFFu];
+                }
+                D.h[cc] += s;
+            }
+            __syncthreads();
+        }
+        // z = M h, stopping test, beta
+        double a2 = 0, c2 = 0;
+        for (int64_t i = threadIdx.x; i < n; i += kB) {
+            const float v = D.h[i], zi = D.invd32[i] * v;
+            D.z[i] = zi;
+            a2 += (double)v * (double)v;
+            c2 += (double)v * (double)zi;
+        }
+        __syncthreads();
+        const double rn2 = bsum(a2, sh), zz = bsum(c2, sh);
+        ++it;
+        if (rn2 < thresh || it >= max_it) break;
+        beta = zz / abs_new;
+        abs_new = zz;
+    }
+    return it;
+}
+
+__global__ __launch_bounds__(kB) void k_fit_batch(const FitDesc* descs) {
+    __shared__ double sh[kBW];
+    const FitDesc& D = descs[blockIdx.x];
+    const int64_t n = D.n;
+    const int64_t max_iter = D.max_iter > 0 ? D.max_iter : 2 * n;
+    if (!D.warm)
+        for (int64_t i = threadIdx.x; i < n; i += kB) D.x64[i] = 0.0;
+    __syncthreads();
+    apply_AT<double>(D, D.b, D.h64);
+    const double rhs2 = dot_self(D.h64, n, sh);
+    int64_t iters = 0, outer = 0;
+    double rel = 0, rel0 = -1;
+    if (rhs2 == 0) {
+        for (int64_t i = threadIdx.x; i < n; i += kB) D.x64[i] = 0.0;
+        __syncthreads();
+    } else {
+        for (;;) {
+            residual64(D, sh);
+            apply_AT<double>(D, D.r64, D.h64);
+            rel = sqrt(dot_self(D.h64, n, sh) / rhs2);
+            if (rel0 < 0) rel0 = rel;
+            if (rel < D.tol || iters >= max_iter) break;
+            double want = fmax(1e-3, 0.5 * D.tol / rel);
+            if (want >= 1.0) want = 0.5;
+            const int64_t it = inner_cgls(D, want, max_iter - iters, sh);
+            iters += it;
+            for (int64_t i = threadIdx.x; i < n; i += kB) D.x64[i] += (double)D.x[i];
+            __syncthreads();
+            ++outer;
+            if (it == 0) break;
+        }
+    }
+    const double rr = residual64(D, sh);
+    const double bb = dot_self(D.b, n, sh);
+    if (threadIdx.x == 0) {
+        D.stats[0] = (double)iters;
+        D.stats[1] = (double)(outer > 0 ? outer - 1 : 0);
+        D.stats[2] = rel;
+        D.stats[3] = bb > 0 ? sqrt(rr / bb) : 0.0;
+        D.stats[4] = sqrt(rr);
+        D.stats[5] = rel0 < 0 ? 0.0 : rel0;
+        D.stats[6] = rel < D.tol ? 1.0 : 0.0;
+    }
+}
+}  // namespace
+}  // namespace its
+
+struct its_batch_s {
+    its_ctx ctx = nullptr;
+    std::vector<its_svo> svos;
+    its::DBuf<its::FitDesc> descs;
+    its::DBuf<double> stats;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<its::FitDesc> host;   // the descriptors the graph was captured with
+    ~its_batch_s() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace its {
+
+its_batch batch_new(its_ctx ctx) {
+    its_batch b = new its_batch_s();
+    b->ctx = ctx;
+    return b;
+}
+void batch_delete(its_batch b) { delete b; }
+void batch_add(its_batch b, its_svo s) {
+    ITS_REQUIRE(s->ctx == b->ctx, ITS_E_INVALID, "octree belongs to another context");
+    ITS_REQUIRE(s->num_verts > 0, ITS_E_STATE, "empty octree");
+    b->svos.push_back(s);
+}
+
+// fit every mesh of the batch in one launch; stats per mesh; seconds: the launch
+void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_stats* out, double* seconds) {
+    const int count = (int)b->svos.size();
+    ITS_REQUIRE(count > 0, ITS_E_STATE, "empty batch");
+    hipStream_t st = b->ctx->stream;
+    std::vector<FitDesc> h(count);
+    b->stats.ensure(8 * (int64_t)count);
+    for (int m = 0; m < count; ++m) {
+        its_svo s = b->svos[m];
+        ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "a mesh of the batch has no right-hand side");
+        ITS_REQUIRE(!warm || s->has_lambda, ITS_E_STATE, "warm batch fit needs coefficients on every mesh");
+        Fit* f = fit_get(s);
+        s->lambda64.ensure(f->n);
+        FitDesc& D = h[m];
+        D = FitDesc{};
+        D.n = f->n;
+        D.L = f->L;
+        D.warm = warm ? 1 : 0;
+        D.max_iter = max_iter;
+        D.tol = tol;
+        for (int d = 0; d <= f->L; ++d) D.off[d] = f->off[d];
+        D.pro_ptr = f->pro_ptr.p;
+        D.pro_idx = f->pro_idx.p;
+        D.res_ptr = f->res_ptr.p;
+        D.res_idx = f->res_idx.p;
+        D.down = f->down.p;
+        D.up = f->up.p;
+        D.finest = f->finest.p;
+        D.invd32 = f->invd32.p;
+        D.b = s->rhs.p;
+        D.x64 = s->lambda64.p;
+        D.x = f->x.p;
+        D.r = f->r.p;
+        D.p = f->p.p;
+        D.z = f->z.p;
+        D.t = f->t.p;
+        D.g = f->g.p;
+        D.h = f->h.p;
+        D.g64 = f->g64.p;
+        D.r64 = f->r64.p;
+        D.h64 = f->h64.p;
+        D.stats = b->stats.p + 8 * m;
+    }
+    const bool same = b->gexec && b->host.size() == h.size() &&
+                      std::equal(h.begin(), h.end(), b->host.begin(),
+                                 [](const FitDesc& x, const FitDesc& y) { return std::memcmp(&x, &y, sizeof x) == 0; });
+    if (!same) {
+        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        b->descs.alloc(count);
+        ITS_HIP(hipMemcpyAsync(b->descs.p, h.data(), count * sizeof(FitDesc), hipMemcpyHostToDevice, st));
+        ITS_HIP(hipStreamSynchronize(st));
+        ITS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(k_fit_batch, dim3(count), dim3(kB), 0, st, (const FitDesc*)b->descs.p);
+        ITS_HIP(hipStreamEndCapture(st, &b->graph));
+        ITS_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->host = h;
+    }
+    ITS_HIP(hipEventRecord(b->ctx->ev0, st));
+    ITS_HIP(hipGraphLaunch(b->gexec, st));
+    ITS_HIP(hipEventRecord(b->ctx->ev1, st));
+    ITS_HIP(hipEventSynchronize(b->ctx->ev1));
+    float ms = 0;
+    ITS_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+    if (seconds) *seconds = ms * 1e-3;
+    std::vector<double> hs(8 * (size_t)count);
+    ITS_HIP(hipMemcpy(hs.data(), b->stats.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int m = 0; m < count; ++m) {
+        if (out) {
+            its_lscg_stats& S = out[m];
+            S.iterations = (int64_t)hs[8 * m];
+            S.refinements = (int64_t)hs[8 * m + 1];
+            S.normal_rel = hs[8 * m + 2];
+            S.residual_rel = hs[8 * m + 3];
+            S.residual_abs = hs[8 * m + 4];
+            S.initial_normal_rel = hs[8 * m + 5];
+            S.converged = hs[8 * m + 6] != 0.0;
+            S.seconds = ms * 1e-3;
+        }
+        b->svos[m]->has_lambda = true;
+        b->svos[m]->field_ready = false;
+    }
+}
+
+// rebuild the evaluation fields after a batch fit (per mesh)
+void batch_fields(its_batch b) {
+    for (its_svo s : b->svos) field_build(s);
+}
+
+int64_t batch_count(its_batch b) { return (int64_t)b->svos.size(); }
+its_svo batch_svo(its_batch b, int64_t m) { return b->svos[(size_t)m]; }
+its_ctx batch_ctx(its_batch b) { return b->ctx; }
+
+}  // namespace its

@@ -3,6 +3,7 @@
 // message for its_last_error(); nothing escapes as an exception or exit().
 #include <cfloat>
 #include <cmath>
+#include <vector>
 #include <cstring>
 #include <new>
 #include <string>
@@ -525,6 +526,69 @@ int its_lattice_f64(its_ctx ctx, its_svo s, double* values) {
     const int64_t D = s->R + 1;
     ITS_HIP(hipMemcpyAsync(values, lat.p, D * D * D * sizeof(double), hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
+    ITS_API_END
+}
+
+// ---- batches
+int its_batch_create(its_ctx ctx, its_batch* out) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx && out, ITS_E_INVALID, "bad arguments");
+    *out = batch_new(ctx);
+    ITS_API_END
+}
+int its_batch_destroy(its_batch b) {
+    ITS_API_BEGIN
+    if (b) {
+        DeviceGuard g(batch_ctx(b)->device);
+        batch_delete(b);
+    }
+    ITS_API_END
+}
+int its_batch_add(its_batch b, its_svo s) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(b, ITS_E_INVALID, "null batch");
+    check_svo(s);
+    batch_add(b, s);
+    ITS_API_END
+}
+int its_batch_fit(its_batch b, double tol, int64_t max_iter, int warm, its_lscg_stats* stats, double* seconds) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(b, ITS_E_INVALID, "null batch");
+    ITS_REQUIRE(tol > 0, ITS_E_INVALID, "tolerance must be positive");
+    DeviceGuard g(batch_ctx(b)->device);
+    batch_fit(b, tol, max_iter, warm != 0, stats, seconds);
+    ITS_API_END
+}
+int its_batch_fields(its_batch b) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(b, ITS_E_INVALID, "null batch");
+    DeviceGuard g(batch_ctx(b)->device);
+    batch_fields(b);
+    ITS_API_END
+}
+int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const float* d_z, const int64_t* offsets,
+                          float out_val, float* d_f, double* seconds) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(b && offsets, ITS_E_INVALID, "bad arguments");
+    const int64_t count = batch_count(b);
+    ITS_REQUIRE(count > 0, ITS_E_STATE, "empty batch");
+    ITS_REQUIRE(offsets[0] == 0, ITS_E_INVALID, "offsets must start at 0");
+    for (int64_t m = 0; m < count; ++m) {
+        ITS_REQUIRE(offsets[m + 1] >= offsets[m], ITS_E_INVALID, "offsets must not decrease");
+        ITS_REQUIRE(batch_svo(b, m)->field_ready, ITS_E_STATE, "a mesh has no evaluation field (its_batch_fields)");
+    }
+    ITS_REQUIRE(offsets[count] == 0 || (d_x && d_y && d_z && d_f), ITS_E_INVALID, "null query arrays");
+    its_ctx ctx = batch_ctx(b);
+    DeviceGuard g(ctx->device);
+    std::vector<its_svo> v(count);
+    for (int64_t m = 0; m < count; ++m) v[m] = batch_svo(b, m);
+    ITS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    eval_batch(v.data(), count, offsets, d_x, d_y, d_z, out_val, d_f, ctx->stream);
+    ITS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ITS_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    ITS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (seconds) *seconds = ms * 1e-3;
     ITS_API_END
 }
 

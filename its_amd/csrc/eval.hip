@@ -311,6 +311,53 @@ __global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams 
     }
 }
 
+// One query through the per-thread walk (the arithmetic of k_eval with one
+// query per thread).
+template <class Loader>
+__device__ inline float eval_query(const Loader& load, const EvalParams& P, int64_t i) {
+    float f = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+    if (!query_front(load, P, i, f, ux, uy, uz)) return f;
+    const int lc = P.lc;
+    const float sc = pow2neg(lc);
+    const uint32_t gcell = (((int)(uz * sc) * P.Rc + (int)(uy * sc)) * P.Rc + (int)(ux * sc));
+    float4 b0 = P.G[2 * (size_t)gcell], b1 = P.G[2 * (size_t)gcell + 1];
+    f = tri_level(b0, b1, ux, uy, uz, sc);
+    if (lc == 0) return f;
+    uint32_t mask = block_mask(b0, b1);
+    const int cx = (int)ux, cy = (int)uy, cz = (int)uz;
+    uint32_t key = child_key(cx, cy, cz, lc - 1);
+    if (!((mask >> key) & 1u)) return f;
+    uint32_t idx = child_index(P.Gbase[gcell], mask, key);
+    for (int d = lc - 1; d >= 0; --d) {
+        b0 = P.blocks[d][2 * (size_t)idx];
+        b1 = P.blocks[d][2 * (size_t)idx + 1];
+        const uint32_t nbase = d > 0 ? P.cbase[d][idx] : 0u;
+        f += tri_level(b0, b1, ux, uy, uz, pow2neg(d));
+        if (d == 0) break;
+        mask = block_mask(b0, b1);
+        key = child_key(cx, cy, cz, d - 1);
+        if (!((mask >> key) & 1u)) break;
+        idx = child_index(nbase, mask, key);
+    }
+    return f;
+}
+
+// Many octrees in one launch: every workgroup takes query ranges of one mesh
+// at a time from the item list (mesh, begin, end).
+struct BatchItem {
+    int64_t begin, end;
+    int32_t mesh, pad;
+};
+__global__ __launch_bounds__(kT) void k_eval_batch(const EvalParams* P, const BatchItem* items, int64_t n_items,
+                                                   const float* x, const float* y, const float* z, float* out) {
+    const SoaF32 ld{x, y, z};
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const BatchItem I = items[it];
+        const EvalParams& p = P[I.mesh];
+        for (int64_t i = I.begin + threadIdx.x; i < I.end; i += blockDim.x) out[i] = eval_query(ld, p, i);
+    }
+}
+
 // Query loop with a per-wavefront queue.
 //
 // Random queries diverge: most stop at the collapsed level, a few walk every
@@ -701,6 +748,35 @@ void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box
     EvalParams P = make_params(s, box_end, mode, out_val);
     if (N <= 0) return;
     launch_eval(AosF64{pts}, s, N, P, f, st);
+}
+
+// queries [off[m], off[m+1]) belong to octree m; support semantics with the
+// octree's own box end (origin + width) and out_val for points outside
+void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
+                const float* z, float out_val, float* f, hipStream_t st) {
+    if (count <= 0 || off[count] <= 0) return;
+    std::vector<EvalParams> P(count);
+    std::vector<BatchItem> items;
+    constexpr int64_t kChunkQ = 16384;
+    for (int64_t m = 0; m < count; ++m) {
+        its_svo s = svos[m];
+        const double end[3] = {s->origin[0] + s->width[0], s->origin[1] + s->width[1], s->origin[2] + s->width[2]};
+        P[m] = make_params(s, end, ITS_EVAL_SUPPORT, out_val);
+        for (int64_t b = off[m]; b < off[m + 1]; b += kChunkQ)
+            items.push_back(BatchItem{b, std::min(off[m + 1], b + kChunkQ), (int32_t)m, 0});
+    }
+    DBuf<EvalParams> dP(count);
+    DBuf<BatchItem> dI((int64_t)items.size());
+    ITS_HIP(hipMemcpyAsync(dP.p, P.data(), count * sizeof(EvalParams), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(dI.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    const its_ctx ctx = svos[0]->ctx;
+    const unsigned grid = (unsigned)std::min<int64_t>((int64_t)items.size(),
+                                                      (int64_t)ctx->prop.multiProcessorCount *
+                                                          tune_get(ITS_TUNE_EVAL_BLOCKS_PER_CU));
+    hipLaunchKernelGGL(k_eval_batch, dim3(grid), dim3(kT), 0, st, (const EvalParams*)dP.p, (const BatchItem*)dI.p,
+                       (int64_t)items.size(), x, y, z, f);
+    ITS_LAUNCH_CHECK();
+    ITS_HIP(hipStreamSynchronize(st));
 }
 
 void eval_lattice(its_svo s, const double o[3], const double vs[3], const int64_t dims[3], float* out, hipStream_t st) {

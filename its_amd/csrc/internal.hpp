@@ -104,6 +104,17 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
                      const double* d_pts, int64_t N, double* d_out, int32_t* d_tri);
 void nearest_normals(hipStream_t st, const double* d_V, const int32_t* d_F, const int32_t* d_tri, int64_t N,
                      float4* d_nrm);
+void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
+                const float* z, float out_val, float* f, hipStream_t st);
+// batch.hip
+its_batch batch_new(its_ctx ctx);
+void batch_delete(its_batch b);
+void batch_add(its_batch b, its_svo s);
+void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_stats* out, double* seconds);
+void batch_fields(its_batch b);
+int64_t batch_count(its_batch b);
+its_svo batch_svo(its_batch b, int64_t m);
+its_ctx batch_ctx(its_batch b);
 // lscg.hip: warm start in the row space of A
 void warm_dual(its_svo s, const double* d_lam, double* d_w);
 void warm_apply(its_svo s, const double* d_w, double* d_lam0, hipStream_t st);

@@ -33,42 +33,10 @@
 #include <cmath>
 
 #include "internal.hpp"
+#include "fit.hpp"
 
 namespace its {
 
-struct Coarse {
-    int L, lc;
-    uint32_t off[kMaxLevels + 1];   // global vertex offsets
-};
-
-struct Fit {
-    int64_t n = 0;
-    int L = 0;
-    int64_t off[kMaxLevels + 1] = {0};
-    DBuf<uint32_t> pro_ptr, pro_idx;   // prolongation: coarse parents of each vertex
-    DBuf<uint32_t> res_ptr, res_idx;   // restriction: fine children of each vertex (| log2 count << 28)
-    DBuf<uint32_t> down, up;           // finest coincident vertex; coincident vertex one level up
-    DBuf<uint8_t> finest;
-    DBuf<float> invd32;
-    DBuf<double> invd64;
-    // inner CGLS vectors (binary32)
-    DBuf<float> x, r, p, z, t, g, h;
-    // binary64 work vectors
-    DBuf<double> g64, h64, r64, nr64, b64, tmp64;
-    // reductions and device scalars
-    DBuf<double> part;
-    DBuf<double> scal;     // see Scal
-    DBuf<int> ctl;         // [0] done, [1] iterations, [2] structure error
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_chunk = 0;
-    int lc = 0;            // first level handled by the one-workgroup coarse kernels
-    Coarse coarse{};
-    ~Fit() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-    }
-};
 void fit_release(Fit* f) { delete f; }
 
 namespace {
@@ -474,6 +442,8 @@ __global__ void k_set_thresh(double* scal, double rel2) { scal[S_THRESH] = rel2 
 // ---------------------------------------------------------------- host side
 unsigned red_blocks(int64_t n) { return (unsigned)std::min<int64_t>(kRedBlocks, blocks_for(n, kT)); }
 
+}  // namespace
+
 Fit* fit_get(its_svo s) {
     if (s->fit) return s->fit.get();
     hipStream_t st = s->ctx->stream;
@@ -569,6 +539,8 @@ Fit* fit_get(its_svo s) {
     s->fit = std::unique_ptr<Fit, void (*)(Fit*)>(f.release(), fit_release);
     return s->fit.get();
 }
+
+namespace {
 
 // y = A x (binary64), via g64
 void apply_A64(Fit* f, const double* x, double* y, hipStream_t st) {
