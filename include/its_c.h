@@ -218,11 +218,12 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo svo, its_mlp net, const double* b, d
 /* ---- many meshes per launch (no reference counterpart: the reference fits
  *      one model per process) ----
  * A batch holds octrees of one context, each with its right-hand side
- * (its_sdf_at_vertices).  its_batch_fit solves all of them in one persistent
- * launch (one workgroup per mesh runs that mesh's whole mixed-precision LSCG,
- * the algorithm and stopping rule of its_lscg_solve), captured once as a
- * hipGraph and replayed on later calls; stats: one per mesh, seconds: the
- * launch.  warm != 0 starts every mesh from its current coefficients.  The
+ * (its_sdf_at_vertices).  its_batch_fit solves the meshes with at most
+ * ITS_TUNE_BATCH_MAX_VERTS unknowns in one persistent launch (one workgroup
+ * per mesh runs that mesh's whole mixed-precision LSCG, the algorithm and
+ * stopping rule of its_lscg_solve), captured once as a hipGraph and replayed
+ * on later calls; larger meshes run at the same time through its_lscg_solve.
+ * stats: one per mesh; seconds: wall time of the fit.  warm != 0 starts every mesh from its current coefficients.  The
  * coefficients stay on the octrees; their evaluation fields are rebuilt by
  * its_batch_fields.  its_batch_eval_device evaluates f for all meshes in one
  * launch: queries [offsets[m], offsets[m+1]) of the device SoA arrays belong
@@ -249,13 +250,16 @@ int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const
  * ITS_TUNE_EVAL_QUEUE: 1 = queries that need the fine octree levels are
  *   gathered per wavefront in LDS and walked 64 at a time; 0 = every thread
  *   walks its own query (default 0).
+ * ITS_TUNE_BATCH_MAX_VERTS: meshes of a batch with at most this many
+ *   unknowns share the one-launch fit (default 65536).
  * No reference counterpart (the reference hard-codes its launch shapes). */
 #define ITS_TUNE_DENSE_CELLS 0
 #define ITS_TUNE_EVAL_BLOCKS_PER_CU 1
 #define ITS_TUNE_COARSE_VERTS 2
 #define ITS_TUNE_EVAL_ILP 3
 #define ITS_TUNE_EVAL_QUEUE 4
-#define ITS_TUNE_COUNT 5
+#define ITS_TUNE_BATCH_MAX_VERTS 5
+#define ITS_TUNE_COUNT 6
 int its_tune_set(int key, int64_t value);
 int its_tune_get(int key, int64_t* value);
 

@@ -256,10 +256,12 @@ struct its_batch_s {
     its::DBuf<double> stats;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    hipStream_t side = nullptr;       // the one-launch fit runs here, beside the large meshes
     std::vector<its::FitDesc> host;   // the descriptors the graph was captured with
     ~its_batch_s() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (side) (void)hipStreamDestroy(side);
     }
 };
 
@@ -277,20 +279,32 @@ void batch_add(its_batch b, its_svo s) {
     b->svos.push_back(s);
 }
 
-// fit every mesh of the batch in one launch; stats per mesh; seconds: the launch
+// Fit every mesh of the batch.  Meshes with at most ITS_TUNE_BATCH_MAX_VERTS
+// unknowns are solved together by the one-launch kernel (graph-captured,
+// replayed on later calls) on the batch's own stream; larger meshes, whose
+// solve on one workgroup would bound the launch, run at the same time on the
+// context stream through the multi-workgroup solver.  seconds: wall time of
+// the whole fit.
 void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_stats* out, double* seconds) {
     const int count = (int)b->svos.size();
     ITS_REQUIRE(count > 0, ITS_E_STATE, "empty batch");
     hipStream_t st = b->ctx->stream;
-    std::vector<FitDesc> h(count);
-    b->stats.ensure(8 * (int64_t)count);
+    const int64_t cap = tune_get(ITS_TUNE_BATCH_MAX_VERTS);
+    std::vector<int> small, big;
     for (int m = 0; m < count; ++m) {
         its_svo s = b->svos[m];
         ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "a mesh of the batch has no right-hand side");
         ITS_REQUIRE(!warm || s->has_lambda, ITS_E_STATE, "warm batch fit needs coefficients on every mesh");
+        (s->num_verts <= cap ? small : big).push_back(m);
+    }
+    const int ns = (int)small.size();
+    std::vector<FitDesc> h(ns);
+    b->stats.ensure(8 * (int64_t)std::max(ns, 1));
+    for (int q = 0; q < ns; ++q) {
+        its_svo s = b->svos[small[q]];
         Fit* f = fit_get(s);
         s->lambda64.ensure(f->n);
-        FitDesc& D = h[m];
+        FitDesc& D = h[q];
         D = FitDesc{};
         D.n = f->n;
         D.L = f->L;
@@ -318,46 +332,59 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
         D.g64 = f->g64.p;
         D.r64 = f->r64.p;
         D.h64 = f->h64.p;
-        D.stats = b->stats.p + 8 * m;
+        D.stats = b->stats.p + 8 * q;
     }
-    const bool same = b->gexec && b->host.size() == h.size() &&
-                      std::equal(h.begin(), h.end(), b->host.begin(),
-                                 [](const FitDesc& x, const FitDesc& y) { return std::memcmp(&x, &y, sizeof x) == 0; });
-    if (!same) {
-        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-        if (b->graph) (void)hipGraphDestroy(b->graph);
-        b->gexec = nullptr;
-        b->graph = nullptr;
-        b->descs.alloc(count);
-        ITS_HIP(hipMemcpyAsync(b->descs.p, h.data(), count * sizeof(FitDesc), hipMemcpyHostToDevice, st));
-        ITS_HIP(hipStreamSynchronize(st));
-        ITS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipLaunchKernelGGL(k_fit_batch, dim3(count), dim3(kB), 0, st, (const FitDesc*)b->descs.p);
-        ITS_HIP(hipStreamEndCapture(st, &b->graph));
-        ITS_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-        b->host = h;
-    }
-    ITS_HIP(hipEventRecord(b->ctx->ev0, st));
-    ITS_HIP(hipGraphLaunch(b->gexec, st));
-    ITS_HIP(hipEventRecord(b->ctx->ev1, st));
-    ITS_HIP(hipEventSynchronize(b->ctx->ev1));
-    float ms = 0;
-    ITS_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
-    if (seconds) *seconds = ms * 1e-3;
-    std::vector<double> hs(8 * (size_t)count);
-    ITS_HIP(hipMemcpy(hs.data(), b->stats.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int m = 0; m < count; ++m) {
-        if (out) {
-            its_lscg_stats& S = out[m];
-            S.iterations = (int64_t)hs[8 * m];
-            S.refinements = (int64_t)hs[8 * m + 1];
-            S.normal_rel = hs[8 * m + 2];
-            S.residual_rel = hs[8 * m + 3];
-            S.residual_abs = hs[8 * m + 4];
-            S.initial_normal_rel = hs[8 * m + 5];
-            S.converged = hs[8 * m + 6] != 0.0;
-            S.seconds = ms * 1e-3;
+    auto t0 = std::chrono::steady_clock::now();
+    if (ns > 0) {
+        if (!b->side) ITS_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+        const bool same = b->gexec && b->host.size() == h.size() &&
+                          std::equal(h.begin(), h.end(), b->host.begin(), [](const FitDesc& x, const FitDesc& y) {
+                              return std::memcmp(&x, &y, sizeof x) == 0;
+                          });
+        if (!same) {
+            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->gexec = nullptr;
+            b->graph = nullptr;
+            b->descs.alloc(ns);
+            ITS_HIP(hipMemcpyAsync(b->descs.p, h.data(), ns * sizeof(FitDesc), hipMemcpyHostToDevice, b->side));
+            ITS_HIP(hipStreamSynchronize(b->side));
+            ITS_HIP(hipStreamBeginCapture(b->side, hipStreamCaptureModeThreadLocal));
+            hipLaunchKernelGGL(k_fit_batch, dim3(ns), dim3(kB), 0, b->side, (const FitDesc*)b->descs.p);
+            ITS_HIP(hipStreamEndCapture(b->side, &b->graph));
+            ITS_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            b->host = h;
         }
+        ITS_HIP(hipStreamSynchronize(st));   // inputs written on the context stream are complete
+        ITS_HIP(hipGraphLaunch(b->gexec, b->side));
+    }
+    std::vector<its_lscg_stats> big_stats(big.size());
+    for (size_t q = 0; q < big.size(); ++q) {
+        its_svo s = b->svos[big[q]];
+        s->lambda64.ensure(s->num_verts);
+        lscg_solve(s, s->rhs.p, s->lambda64.p, tol, max_iter, warm, &big_stats[q]);
+    }
+    if (ns > 0) ITS_HIP(hipStreamSynchronize(b->side));
+    ITS_HIP(hipStreamSynchronize(st));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (seconds) *seconds = secs;
+    std::vector<double> hs(8 * (size_t)std::max(ns, 1));
+    if (ns > 0) ITS_HIP(hipMemcpy(hs.data(), b->stats.p, 8 * (size_t)ns * sizeof(double), hipMemcpyDeviceToHost));
+    for (int q = 0; q < ns; ++q) {
+        if (!out) break;
+        its_lscg_stats& S = out[small[q]];
+        S.iterations = (int64_t)hs[8 * q];
+        S.refinements = (int64_t)hs[8 * q + 1];
+        S.normal_rel = hs[8 * q + 2];
+        S.residual_rel = hs[8 * q + 3];
+        S.residual_abs = hs[8 * q + 4];
+        S.initial_normal_rel = hs[8 * q + 5];
+        S.converged = hs[8 * q + 6] != 0.0;
+        S.seconds = secs;
+    }
+    for (size_t q = 0; q < big.size(); ++q)
+        if (out) out[big[q]] = big_stats[q];
+    for (int m = 0; m < count; ++m) {
         b->svos[m]->has_lambda = true;
         b->svos[m]->field_ready = false;
     }

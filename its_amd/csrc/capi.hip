@@ -14,7 +14,7 @@ namespace its {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
-static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 64, 2048, 1, 0};
+static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 64, 2048, 1, 0, 65536};
 int64_t tune_get(int key) { return g_tune[key]; }
 }  // namespace its
 

@@ -145,7 +145,8 @@ def torus_knot_tube(n_seg: int = 2000, n_ring: int = 50, p: int = 3, q: int = 5,
     v10 = ((i + 1) % n_seg) * n_ring + j
     v01 = i * n_ring + (j + 1) % n_ring
     v11 = ((i + 1) % n_seg) * n_ring + (j + 1) % n_ring
-    F = np.concatenate([np.stack([v00, v10, v11], -1).reshape(-1, 3), np.stack([v00, v11, v01], -1).reshape(-1, 3)])
+    # counter-clockwise seen from outside: outward normals, winding number 1 inside
+    F = np.concatenate([np.stack([v00, v11, v10], -1).reshape(-1, 3), np.stack([v00, v01, v11], -1).reshape(-1, 3)])
     return V, F.astype(np.int32)
 
 

@@ -29,9 +29,13 @@ def _build(ctx):
     return out
 
 
-def test_batch_fit_and_eval_match_single(ctx):
+@pytest.mark.parametrize("cap", [1 << 30, 3000])
+def test_batch_fit_and_eval_match_single(ctx, cap):
+    """cap: meshes above it take the multi-workgroup solver beside the launch."""
     import torch
     from its_amd import _capi
+    old_cap = _capi.tune_get(_capi.TUNE_BATCH_MAX_VERTS)
+    _capi.tune_set(_capi.TUNE_BATCH_MAX_VERTS, cap)
     items = _build(ctx)
     print("[batch] meshes", [f"{c.name}/{c.R} n={g.num_verts}" for c, g, _ in items], flush=True)
     single = []
@@ -71,6 +75,7 @@ def test_batch_fit_and_eval_match_single(ctx):
             f_single = g.eval(q, c.origin + c.width, _capi.EVAL_SUPPORT, 0.5)
             assert np.array_equal(fb[offs[m]:offs[m + 1]], f_single), c
     batch.close()
+    _capi.tune_set(_capi.TUNE_BATCH_MAX_VERTS, old_cap)
 
 
 def test_batch_warm_and_errors(ctx):
