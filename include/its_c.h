@@ -151,6 +151,19 @@ int its_bspline_eval_f64pts(its_ctx ctx, its_svo svo, const double* pts, int64_t
 /* device SoA coordinates x[N], y[N], z[N]; f[N] */
 int its_bspline_eval_device(its_ctx ctx, its_svo svo, const float* d_x, const float* d_y, const float* d_z, int64_t N,
                             const double box_end[3], int mode, float out_val, float* d_f, void* stream);
+/* ---- shell iso band (ThinShells::cpIntersectionPoints, ThinShells.cpp:38-479,
+ *      and the iso values of cpLatentBSplineValue, 655-755) ----
+ * Points where the mesh edges cross the back/left/bottom faces of level-0
+ * nodes, then points where level-0 node edges cross the triangles (edge
+ * points first; duplicates are kept, order is deterministic).  pts may be NULL
+ * to count; capacity is in points. */
+int its_shell_points(its_ctx ctx, its_svo svo, const double* V, int64_t nV, const int32_t* F, int64_t nF,
+                     double* pts, int64_t capacity, int64_t* n_edge, int64_t* n_face);
+/* inner / outer iso values: min and max of f over the model vertices and
+ * those points (needs the fitted field); n_points: points evaluated */
+int its_shell_iso(its_ctx ctx, its_svo svo, const double* V, int64_t nV, const int32_t* F, int64_t nF,
+                  double iso[2], int64_t* n_points);
+
 /* {-1, 0, 1} classification of f (ThinShells.cpp:1412-1417) */
 int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, int8_t* cls);
 

@@ -114,6 +114,9 @@ SIGNATURES = {
     "its_timer_start": [vp],
     "its_timer_stop": [vp, C.POINTER(C.c_double)],
     "its_lscg_diag": [vp, vp, _f64p],
+    "its_shell_points": [vp, vp, _f64p, C.c_int64, _i32p, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64),
+                         C.POINTER(C.c_int64)],
+    "its_shell_iso": [vp, vp, _f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.POINTER(C.c_int64)],
     "its_lattice_f64": [vp, vp, _f64p],
     "its_batch_create": [vp, C.POINTER(vp)],
     "its_batch_destroy": [vp],
@@ -365,6 +368,27 @@ class Octree:
         d = st.as_dict()
         d["seconds_predict"] = tp.value
         return lam, d
+
+    def shell_points(self, V, F):
+        """(points, n_edge): mesh-edge x node-face points, then node-edge x face points."""
+        V, F = _f64(V), np.ascontiguousarray(F, np.int32)
+        ne, nf = C.c_int64(0), C.c_int64(0)
+        check(lib().its_shell_points(self.ctx.handle, self._h, V.reshape(-1), len(V), F.reshape(-1), len(F), None, 0,
+                                     C.byref(ne), C.byref(nf)))
+        n = ne.value + nf.value
+        out = np.zeros((max(n, 1), 3))
+        check(lib().its_shell_points(self.ctx.handle, self._h, V.reshape(-1), len(V), F.reshape(-1), len(F),
+                                     out.ctypes.data_as(vp), n, C.byref(ne), C.byref(nf)))
+        return out[:n], ne.value
+
+    def shell_iso(self, V, F):
+        """(inner, outer) iso values and the number of intersection points."""
+        V, F = _f64(V), np.ascontiguousarray(F, np.int32)
+        iso = np.zeros(2)
+        n = C.c_int64(0)
+        check(lib().its_shell_iso(self.ctx.handle, self._h, V.reshape(-1), len(V), F.reshape(-1), len(F), iso,
+                                  C.byref(n)))
+        return float(iso[0]), float(iso[1]), n.value
 
     def lscg_diag(self):
         d = np.zeros(self.num_verts)

@@ -529,6 +529,79 @@ int its_lattice_f64(its_ctx ctx, its_svo s, double* values) {
     ITS_API_END
 }
 
+// ---- shell iso band
+namespace {
+struct ShellInput {
+    DBuf<double> V, tris;
+    DBuf<uint64_t> edges;
+    int64_t ne = 0;
+};
+ShellInput shell_upload(const double* V, int64_t nV, const int32_t* F, int64_t nF, hipStream_t st) {
+    ITS_REQUIRE(V && F && nV > 0 && nF > 0, ITS_E_INVALID, "empty mesh");
+    for (int64_t i = 0; i < 3 * nF; ++i)
+        ITS_REQUIRE(F[i] >= 0 && F[i] < nV, ITS_E_INVALID, "face index out of range");
+    ShellInput in;
+    std::vector<uint64_t> e = mesh_edges(F, nF);
+    std::vector<double> t(9 * (size_t)nF);
+    for (int64_t f = 0; f < nF; ++f)
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) t[9 * f + 3 * c + a] = V[3 * (int64_t)F[3 * f + c] + a];
+    in.V = upload(V, 3 * (size_t)nV, st);
+    in.tris = upload(t.data(), t.size(), st);
+    in.edges = upload(e.data(), e.size(), st);
+    in.ne = (int64_t)e.size();
+    return in;
+}
+}  // namespace
+
+int its_shell_points(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* pts,
+                     int64_t capacity, int64_t* n_edge, int64_t* n_face) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    ShellInput in = shell_upload(V, nV, F, nF, st);
+    DBuf<double> P;
+    int64_t ne = 0, nf = 0;
+    shell_points(s, in.V.p, in.edges.p, in.ne, in.tris.p, nF, P, &ne, &nf, st);
+    if (n_edge) *n_edge = ne;
+    if (n_face) *n_face = nf;
+    if (pts && ne + nf > 0) {
+        ITS_REQUIRE(ne + nf <= capacity, ITS_E_RANGE, "point buffer too small");
+        ITS_HIP(hipMemcpyAsync(pts, P.p, 3 * (ne + nf) * sizeof(double), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    ITS_API_END
+}
+
+int its_shell_iso(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double iso[2],
+                  int64_t* n_points) {
+    ITS_API_BEGIN
+    check_svo(s);
+    ITS_REQUIRE(ctx == s->ctx && iso, ITS_E_INVALID, "bad arguments");
+    ITS_REQUIRE(s->field_ready, ITS_E_STATE, "no coefficient field: run a fit or set lambda first");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    ShellInput in = shell_upload(V, nV, F, nF, st);
+    DBuf<double> P;
+    int64_t ne = 0, nf = 0;
+    shell_points(s, in.V.p, in.edges.p, in.ne, in.tris.p, nF, P, &ne, &nf, st);
+    const int64_t np = ne + nf, nall = nV + np;
+    DBuf<float> f(nall);
+    const double dummy[3] = {0, 0, 0};
+    eval_points_aos64(s, in.V.p, nV, dummy, ITS_EVAL_RAW, 0.f, f.p, st);
+    if (np > 0) eval_points_aos64(s, P.p, np, dummy, ITS_EVAL_RAW, 0.f, f.p + nV, st);
+    std::vector<float> h(nall);
+    ITS_HIP(hipMemcpyAsync(h.data(), f.p, nall * sizeof(float), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    const auto mm = std::minmax_element(h.begin(), h.end());
+    iso[0] = *mm.first;
+    iso[1] = *mm.second;
+    if (n_points) *n_points = np;
+    ITS_API_END
+}
+
 // ---- batches
 int its_batch_create(its_ctx ctx, its_batch* out) {
     ITS_API_BEGIN

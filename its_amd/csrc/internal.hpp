@@ -1,6 +1,7 @@
 // Internal object layout of the ITS gfx950 library (not part of the C ABI).
 #pragma once
 #include <memory>
+#include <vector>
 
 #include "common.hpp"
 
@@ -106,6 +107,10 @@ void nearest_normals(hipStream_t st, const double* d_V, const int32_t* d_F, cons
                      float4* d_nrm);
 void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
                 const float* z, float out_val, float* f, hipStream_t st);
+// shell.hip
+void shell_points(its_svo s, const double* d_V, const uint64_t* d_edges, int64_t ne, const double* d_tris, int64_t nT,
+                  DBuf<double>& pts, int64_t* n_edge, int64_t* n_face, hipStream_t st);
+std::vector<uint64_t> mesh_edges(const int32_t* F, int64_t nF);
 // batch.hip
 its_batch batch_new(its_ctx ctx);
 void batch_delete(its_batch b);

@@ -14,6 +14,9 @@
 //     within 1e-4 of 0 or 1.
 // Brute force over triangles: each 256-thread block stages 256 triangles at
 // a time in LDS, and every thread tests its point against the staged tile.
+// The winding number needs every triangle; the binary64 closest point is
+// computed only for triangles whose bounding sphere can beat the distance
+// found so far (see k_sdf), which leaves the result unchanged.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -79,28 +82,88 @@ __device__ inline float solid_angle_f(const float p[3], const double* A, const d
     return 2.0f * atan2f(det, den);
 }
 
-__global__ __launch_bounds__(kT) void k_sdf(const double* tris, int64_t nF, const double* pts, int64_t N, double* out,
-                                           int32_t* tri_out) {
+// Per-triangle bounding sphere in binary32: centre = vertex mean, radius =
+// largest centre-vertex distance, both rounded outward by a margin that
+// covers the binary32 rounding of every quantity derived from them.
+__device__ inline float bound_margin(float scale) { return 1e-5f * scale + 1e-30f; }
+
+__global__ void k_tri_spheres(const double* tris, int64_t nF, float4* sph) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nF) return;
+    const double* T = tris + 9 * t;
+    double c[3];
+    for (int a = 0; a < 3; ++a) c[a] = (T[a] + T[3 + a] + T[6 + a]) / 3.0;
+    double r2 = 0, cm = 0;
+    for (int v = 0; v < 3; ++v) {
+        double d2 = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double d = T[3 * v + a] - c[a];
+            d2 += d * d;
+        }
+        r2 = fmax(r2, d2);
+    }
+    for (int a = 0; a < 3; ++a) cm = fmax(cm, fabs(c[a]));
+    const double r = sqrt(r2);
+    sph[t] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)(r * (1.0 + 1e-6) + 1e-6 * (cm + r)));
+}
+
+// Nearest triangle in binary64 (Ericson's closest point) and the winding
+// number (binary32 solid angles), brute force over LDS tiles of triangles.
+// A first pass over the triangles' bounding spheres gives an upper bound on
+// the point's distance; the second pass computes the exact binary64 distance
+// only for triangles whose sphere can come closer than the running best.  A
+// triangle it skips is strictly farther than the nearest one, so the minimum
+// and the index of its first occurrence are those of the full search.
+__global__ __launch_bounds__(kT) void k_sdf(const double* tris, const float4* sph, int64_t nF, const double* pts,
+                                           int64_t N, double* out, int32_t* tri_out) {
     __shared__ double sh[kT * 9];
+    __shared__ float4 ss[kT];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double p[3] = {0, 0, 0};
     if (i < N) { p[0] = pts[3 * i]; p[1] = pts[3 * i + 1]; p[2] = pts[3 * i + 2]; }
     float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
-    double best = 1e300;
-    int64_t best_t = -1;
-    float wsum = 0.f;
+    const float pm = fmaxf(fabsf(pf[0]), fmaxf(fabsf(pf[1]), fabsf(pf[2])));
+    // pass 1: upper bound of the distance from the bounding spheres
+    float ub = 3.0e38f;
     for (int64_t base = 0; base < nF; base += kT) {
         __syncthreads();
         int64_t t = base + threadIdx.x;
-        if (t < nF)
+        if (t < nF) ss[threadIdx.x] = sph[t];
+        __syncthreads();
+        const int cnt = (int)std::min<int64_t>(kT, nF - base);
+        if (i < N)
+            for (int k = 0; k < cnt; ++k) {
+                const float4 S = ss[k];
+                const float dx = pf[0] - S.x, dy = pf[1] - S.y, dz = pf[2] - S.z;
+                const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
+                ub = fminf(ub, d + S.w + m);
+            }
+    }
+    double best = (ub < 3.0e38f) ? (double)ub * (double)ub * (1.0 + 1e-6) : 1e300;
+    int64_t best_t = -1;
+    float wsum = 0.f;
+    // pass 2: exact distance where it can matter, and the winding number
+    for (int64_t base = 0; base < nF; base += kT) {
+        __syncthreads();
+        int64_t t = base + threadIdx.x;
+        if (t < nF) {
             for (int k = 0; k < 9; ++k) sh[threadIdx.x * 9 + k] = tris[9 * t + k];
+            ss[threadIdx.x] = sph[t];
+        }
         __syncthreads();
         int cnt = (int)std::min<int64_t>(kT, nF - base);
         if (i < N) {
             for (int k = 0; k < cnt; ++k) {
                 const double* T = &sh[9 * k];
-                double d2 = point_tri_dist2(p, T, T + 3, T + 6);
-                if (d2 < best) { best = d2; best_t = base + k; }
+                const float4 S = ss[k];
+                const float dx = pf[0] - S.x, dy = pf[1] - S.y, dz = pf[2] - S.z;
+                const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
+                const float lb = sqrtf(dx * dx + dy * dy + dz * dz) - S.w - m;
+                if (!(lb > 0.f && (double)lb * (double)lb > best)) {
+                    const double d2 = point_tri_dist2(p, T, T + 3, T + 6);
+                    if (d2 < best) { best = d2; best_t = base + k; }
+                }
                 wsum += solid_angle_f(pf, T, T + 3, T + 6);
             }
         }
@@ -126,11 +189,15 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
     (void)nV;
     if (N <= 0) return;
     DBuf<double> tris(std::max<int64_t>(9 * nF, 9));
+    DBuf<float4> sph(std::max<int64_t>(nF, 1));
     if (nF > 0) {
         hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, d_V, d_F, nF, tris.p);
         ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_tri_spheres, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, (const double*)tris.p, nF, sph.p);
+        ITS_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_sdf, dim3(blocks_for(N, kT)), dim3(kT), 0, st, tris.p, nF, d_pts, N, d_out, d_tri);
+    hipLaunchKernelGGL(k_sdf, dim3(blocks_for(N, kT)), dim3(kT), 0, st, tris.p, (const float4*)sph.p, nF, d_pts, N,
+                       d_out, d_tri);
     ITS_LAUNCH_CHECK();
     ITS_HIP(hipStreamSynchronize(st));
 }

@@ -5,7 +5,8 @@
 //   SparseVoxelOctree::createOctree      (reference src/SVO.h:104)
 //   ThinShells(filename, gx, gy, gz, is2UnitCube, scaleFactor)
 //                                         (reference src/ThinShells.h:67-77)
-//   ThinShells::creatShell               (reference src/ThinShells.cpp:757-815)
+//   ThinShells::creatShell               (reference src/ThinShells.cpp:757-815,
+//                                         cpIntersectionPoints 38-479)
 //   ThinShells::getShellIsoVal           (reference src/ThinShells.h:126)
 //   ThinShells::multiPointQuery          (reference src/ThinShells.cpp:1183-1420)
 //   ThinShells::mcVisualization          (reference src/ThinShells.cpp:1086-1129)
@@ -15,10 +16,10 @@
 // as its_host::Error.  Header only; link with its_amd/libits_hip.so.
 //
 // Differences, by design:
-//   - creatShell computes the right-hand side, the fit and f at the model
-//     vertices; the intersection points of cpIntersectionPoints are not part
-//     of this path, so the shell iso values are the min / max of f over the
-//     model vertices only.
+//   - creatShell computes the right-hand side, the fit, f at the model
+//     vertices and the shell iso values over the model vertices and the
+//     intersection points (its_shell_iso); the intersection points
+//     themselves stay on the device.
 //   - MC::marching_cubes takes the octree object (which holds the vertex
 //     arrays and the coefficients on the device) instead of host copies of
 //     nodeVertexArray, nodeWidthArray and lambda.
@@ -230,8 +231,12 @@ public:
         const double end[3] = {bbox.boxEnd[0], bbox.boxEnd[1], bbox.boxEnd[2]};
         check(its_bspline_eval_f64pts(c, svo.handle(), mV.data(), nv, end, ITS_EVAL_SUPPORT, 0.f, f.data()));
         bSplineVal.assign(f.begin(), f.end());
-        innerShellIsoVal = *std::min_element(bSplineVal.begin(), bSplineVal.end());
-        outerShellIsoVal = *std::max_element(bSplineVal.begin(), bSplineVal.end());
+        double iso[2];
+        int64_t npts = 0;
+        check(its_shell_iso(c, svo.handle(), mV.data(), nv, mF.data(), (int64_t)(mF.size() / 3), iso, &npts));
+        innerShellIsoVal = iso[0];
+        outerShellIsoVal = iso[1];
+        numInterPoints = npts;
     }
 
     std::array<double, 2> getShellIsoVal() const { return {innerShellIsoVal, outerShellIsoVal}; }
@@ -275,6 +280,7 @@ public:
     const AABox<V3d>& boundingBox() const { return bbox; }
     const SparseVoxelOctree& octree() const { return svo; }
     double tol = 1e-6;   // stopping tolerance of the fit (normal-equation residual)
+    int64_t numInterPoints = 0;   // intersection points behind the iso values
 
 private:
     static constexpr double kInf = std::numeric_limits<double>::max();

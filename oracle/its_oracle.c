@@ -1299,3 +1299,208 @@ void ora_signed_distance(const double* V, int64_t nV, const int32_t* F, int64_t 
         out[i] = (w > 0.5) ? -d : d;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Shell iso band: intersection points (ThinShells.cpp:38-479)               */
+/* ------------------------------------------------------------------------ */
+typedef struct { double a[3], b[3]; } seg;
+
+static int cmp_u64(const void* x, const void* y) {
+    uint64_t a = *(const uint64_t*)x, b = *(const uint64_t*)y;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+/* qsort form of the fuzzy std::less<V3d> */
+static int cmp_point(const void* x, const void* y) {
+    const int l = less_point((const double*)x, (const double*)y);
+    return l == 1 ? -1 : (l == -1 ? 1 : 0);
+}
+/* std::less<V3d> of two segments: first endpoints, then second (SVO.cu:1033-1052) */
+static int cmp_seg(const void* x, const void* y) {
+    const seg* a = (const seg*)x;
+    const seg* b = (const seg*)y;
+    if (less_point(a->a, b->a) == 1) return -1;
+    if (less_point(b->a, a->a) == 1) return 1;
+    if (less_point(a->b, b->b) == 1) return -1;
+    if (less_point(b->b, a->b) == 1) return 1;
+    return 0;
+}
+static int g_axis; /* sort key axis of the edge sweeps */
+/* x_sortEdge / y_sortEdge / z_sortEdge (ThinShells.cpp:170-209): by the end
+ * point's sweep coordinate, then the other two in the order of the reference */
+static int cmp_end(const void* x, const void* y) {
+    const double* a = ((const seg*)x)->b;
+    const double* b = ((const seg*)y)->b;
+    static const int order[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+    for (int k = 0; k < 3; ++k) {
+        int i = order[g_axis][k];
+        if (fabs(a[i] - b[i]) < 1e-9) continue;
+        return a[i] < b[i] ? -1 : 1;
+    }
+    return 0;
+}
+static inline double dot3l(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+typedef struct { double* p; int64_t n, cap; } pbuf;
+static void push3(pbuf* b, const double* q) {
+    if (b->n == b->cap) { b->cap = b->cap ? 2 * b->cap : 1024; b->p = (double*)realloc(b->p, 3 * sizeof(double) * b->cap); }
+    memcpy(b->p + 3 * b->n, q, 3 * sizeof(double));
+    b->n++;
+}
+
+int64_t ora_intersections(const ora_svo* s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* out,
+                          int64_t cap, int64_t* n_edge) {
+    (void)nV;
+    const int R = s->R;
+    const double* mo = s->grid_origin;
+    const double vw = s->voxel_width;
+    const int64_t n0 = s->depth_num_nodes[0];
+    pbuf E = {0, 0, 0}, Fp = {0, 0, 0};
+    /* model edges: the ordered set of (min, max) vertex pairs (BaseModel.cpp:291-313) */
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * 3 * (nF > 0 ? nF : 1));
+    for (int64_t f = 0; f < nF; ++f) {
+        int32_t i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];
+        int32_t mx = i0 > i1 ? (i0 > i2 ? i0 : i2) : (i1 > i2 ? i1 : i2);
+        int32_t mn = i0 < i1 ? (i0 < i2 ? i0 : i2) : (i1 < i2 ? i1 : i2);
+        int32_t md = i0 + i1 + i2 - mx - mn;
+        keys[3 * f] = ((uint64_t)(uint32_t)mn << 32) | (uint32_t)md;
+        keys[3 * f + 1] = ((uint64_t)(uint32_t)md << 32) | (uint32_t)mx;
+        keys[3 * f + 2] = ((uint64_t)(uint32_t)mn << 32) | (uint32_t)mx;
+    }
+    qsort(keys, 3 * nF, sizeof(uint64_t), cmp_u64);
+    int64_t ne = 0;
+    for (int64_t k = 0; k < 3 * nF; ++k)
+        if (k == 0 || keys[k] != keys[k - 1]) keys[ne++] = keys[k];
+    /* mesh edges x back/left/bottom faces of level-0 nodes (ThinShells.cpp:65-138) */
+    for (int64_t k = 0; k < ne; ++k) {
+        const double* p1 = V + 3 * (int64_t)(keys[k] >> 32);
+        const double* p2 = V + 3 * (int64_t)(keys[k] & 0xFFFFFFFFu);
+        double dir[3];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            dir[a] = p2[a] - p1[a];
+            int d1 = (int)((p1[a] - mo[a]) / vw), d2 = (int)((p2[a] - mo[a]) / vw);
+            int l = (d1 < d2 ? d1 : d2) - 1, h = (d1 > d2 ? d1 : d2) + 1;
+            lo[a] = l < 0 ? 0 : (l > R - 1 ? R - 1 : l);
+            hi[a] = h < 0 ? 0 : (h > R - 1 ? R - 1 : h);
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    int64_t node = find_node(s, 0, ora_morton_encode((uint32_t)x, (uint32_t)y, (uint32_t)z));
+                    if (node < 0) continue;
+                    const double* o = s->origin + 3 * node;
+                    const double w = s->width[node];
+                    for (int a = 0; a < 3; ++a) {   /* back (x), left (y), bottom (z) planes */
+                        double t = DBL_MAX;
+                        if (dir[a] != 0) t = (o[a] - p1[a]) / dir[a];
+                        if (!(0.0 <= t && t <= 1.0)) continue;
+                        double q[3] = {p1[0] + t * dir[0], p1[1] + t * dir[1], p1[2] + t * dir[2]};
+                        int b = (a + 1) % 3, c = (a + 2) % 3;
+                        if (b > c) { int tmp = b; b = c; c = tmp; }
+                        if (o[b] <= q[b] && q[b] <= o[b] + w && o[c] <= q[c] && q[c] <= o[c] + w) push3(&E, q);
+                    }
+                }
+    }
+    free(keys);
+    qsort(E.p, E.n, 3 * sizeof(double), cmp_point);
+    int64_t nu = 0;
+    for (int64_t k = 0; k < E.n; ++k)
+        if (nu == 0 || memcmp(E.p + 3 * k, E.p + 3 * (nu - 1), 3 * sizeof(double)) != 0) {
+            memmove(E.p + 3 * nu, E.p + 3 * k, 3 * sizeof(double));
+            ++nu;
+        }
+    E.n = nu;
+    /* fine node edges (determineNodeEdge, SVO.cu:965-1008), sorted and unique */
+    static const int eo[12][2][3] = {
+        {{0, 0, 0}, {0, 1, 0}}, {{0, 1, 0}, {1, 1, 0}}, {{1, 0, 0}, {1, 1, 0}}, {{0, 0, 0}, {1, 0, 0}},
+        {{0, 0, 1}, {0, 1, 1}}, {{0, 1, 1}, {1, 1, 1}}, {{1, 0, 1}, {1, 1, 1}}, {{0, 0, 1}, {1, 0, 1}},
+        {{0, 0, 0}, {0, 0, 1}}, {{0, 1, 0}, {0, 1, 1}}, {{1, 1, 0}, {1, 1, 1}}, {{1, 0, 0}, {1, 0, 1}}};
+    seg* S = (seg*)malloc(sizeof(seg) * 12 * (n0 > 0 ? n0 : 1));
+    for (int64_t i = 0; i < n0; ++i) {
+        const double* o = s->origin + 3 * (s->esum_nodes[0] + i);
+        const double w = s->width[s->esum_nodes[0] + i];
+        for (int e = 0; e < 12; ++e)
+            for (int a = 0; a < 3; ++a) {
+                S[12 * i + e].a[a] = o[a] + (eo[e][0][a] ? w : 0.0);
+                S[12 * i + e].b[a] = o[a] + (eo[e][1][a] ? w : 0.0);
+            }
+    }
+    qsort(S, 12 * n0, sizeof(seg), cmp_seg);
+    int64_t ns = 0;
+    for (int64_t k = 0; k < 12 * n0; ++k)
+        if (ns == 0 || cmp_seg(&S[k], &S[ns - 1]) != 0) S[ns++] = S[k];
+    /* per axis, sorted by the end point (ThinShells.cpp:211-224) */
+    seg* A[3];
+    int64_t na[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+        A[a] = (seg*)malloc(sizeof(seg) * (ns > 0 ? ns : 1));
+        for (int64_t k = 0; k < ns; ++k) {
+            int axis = fabs(S[k].a[0] - S[k].b[0]) > 1e-12 ? 0 : (fabs(S[k].a[1] - S[k].b[1]) > 1e-12 ? 1 : 2);
+            if (axis == a) A[a][na[a]++] = S[k];
+        }
+        g_axis = a;
+        qsort(A[a], na[a], sizeof(seg), cmp_end);
+    }
+    free(S);
+    /* mesh faces x node edges, swept per axis (ThinShells.cpp:242-338) */
+    for (int64_t f = 0; f < nF; ++f) {
+        const double* p1 = V + 3 * (int64_t)F[3 * f];
+        const double* p2 = V + 3 * (int64_t)F[3 * f + 1];
+        const double* p3 = V + 3 * (int64_t)F[3 * f + 2];
+        double e1[3], e2[3], e3[3], u[3], v[3], n[3], bo[3], be[3];
+        for (int a = 0; a < 3; ++a) {
+            e1[a] = p2[a] - p1[a];
+            e2[a] = p3[a] - p2[a];
+            e3[a] = p1[a] - p3[a];
+            u[a] = p2[a] - p1[a];
+            v[a] = p3[a] - p1[a];
+            bo[a] = fmin(p1[a], fmin(p2[a], p3[a]));
+            be[a] = fmax(p1[a], fmax(p2[a], p3[a]));
+        }
+        cross3(u, v, n);
+        const double mn[3] = {-n[0], -n[1], -n[2]};
+        const double tdir = dot3l(mn, p1);
+        for (int a = 0; a < 3; ++a) {
+            /* lower_bound: first edge whose end coordinate is not below bo[a] by more than 1e-9 */
+            int64_t lo = 0, hi = na[a];
+            while (lo < hi) {
+                int64_t mid = (lo + hi) / 2;
+                double ea = A[a][mid].b[a];
+                int less = (fabs(bo[a] - ea) >= 1e-9 && bo[a] > ea);
+                if (less) lo = mid + 1; else hi = mid;
+            }
+            for (int64_t k = lo; k < na[a]; ++k) {
+                const double* q1 = A[a][k].a;
+                const double* q2 = A[a][k].b;
+                if (fabs(q1[a] - be[a]) >= 1e-9 && q1[a] > be[a]) break;
+                double ed[3] = {q2[0] - q1[0], q2[1] - q1[1], q2[2] - q1[2]};
+                const double nd = dot3l(n, ed);
+                if (fabsf((float)nd) < 1e-9f) continue;
+                const double t = (-tdir - dot3l(n, q1)) / nd;
+                if (t < 0. || t > 1.) continue;
+                double ip[3] = {q1[0] + ed[0] * t, q1[1] + ed[1] * t, q1[2] + ed[2] * t};
+                double w1[3], w2[3], w3[3], c[3];
+                for (int b = 0; b < 3; ++b) { w1[b] = ip[b] - p1[b]; w2[b] = ip[b] - p2[b]; w3[b] = ip[b] - p3[b]; }
+                cross3(e1, w1, c); if (dot3l(c, n) < 0) continue;
+                cross3(e2, w2, c); if (dot3l(c, n) < 0) continue;
+                cross3(e3, w3, c); if (dot3l(c, n) < 0) continue;
+                push3(&Fp, ip);
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a) free(A[a]);
+    const int64_t total = E.n + Fp.n;
+    if (n_edge) *n_edge = E.n;
+    if (out && total <= cap) {
+        memcpy(out, E.p, 3 * sizeof(double) * E.n);
+        memcpy(out + 3 * E.n, Fp.p, 3 * sizeof(double) * Fp.n);
+    }
+    free(E.p);
+    free(Fp.p);
+    return total;
+}

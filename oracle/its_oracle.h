@@ -116,6 +116,14 @@ void ora_signed_distance(const double* V, int64_t nV, const int32_t* F, int64_t 
                          const double* pts, int64_t n, double* out);
 double ora_winding_number(const double* V, const int32_t* F, int64_t nF, const double p[3]);
 
+/* ---- shell iso band (ThinShells::cpIntersectionPoints, ThinShells.cpp:38-479) ----
+ * Points where mesh edges cross the back/left/bottom faces of level-0 nodes
+ * (sorted with the fuzzy order and exact duplicates removed), then points
+ * where level-0 node edges cross mesh triangles.  Writes up to cap points
+ * (x, y, z) and returns the total count; n_edge receives the edge points. */
+int64_t ora_intersections(const ora_svo* s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* out,
+                          int64_t cap, int64_t* n_edge);
+
 int ora_num_threads(void);
 
 #ifdef __cplusplus

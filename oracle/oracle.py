@@ -75,6 +75,7 @@ def lib():
             "ora_signed_distance": (None, [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, _f64p]),
             "ora_winding_number": (C.c_double, [_f64p, _i32p, C.c_int64, _f64p]),
             "ora_num_threads": (C.c_int, []),
+            "ora_intersections": (C.c_int64, [vp, _f64p, C.c_int64, _i32p, C.c_int64, vp, C.c_int64, _i64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -214,6 +215,19 @@ class Matrix:
         stats = {"iterations": int(st.view(np.int64)[0]), "normal_rel": st[1], "residual_rel": st[2],
                  "residual_abs": st[3]}
         return x, stats
+
+
+def intersections(svo: SVO, V, F):
+    """Shell iso-band points (ThinShells::cpIntersectionPoints): unique
+    mesh-edge x node-face points, then node-edge x mesh-face points."""
+    V = _f64(V)
+    F = np.ascontiguousarray(F, np.int32)
+    ne = np.zeros(1, np.int64)
+    n = lib().ora_intersections(svo._h, V.reshape(-1), len(V), F.reshape(-1), len(F), None, 0, ne)
+    out = np.zeros((max(n, 1), 3))
+    lib().ora_intersections(svo._h, V.reshape(-1), len(V), F.reshape(-1), len(F), out.ctypes.data_as(C.c_void_p), n,
+                            ne)
+    return out[:n], int(ne[0])
 
 
 def lscg_operator(mul, tmul, invdiag, b, tol=1e-6, max_iter=None, x0=None):

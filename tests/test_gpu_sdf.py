@@ -31,3 +31,28 @@ def test_sdf_sphere_closed_form(ctx):
     assert np.all(np.abs(d - (r - 1)) < 5e-3)
     far = np.abs(r - 1) > 0.01
     assert np.all(np.sign(d[far]) == np.sign(r[far] - 1))
+
+
+def test_sdf_culling_keeps_exact_results(ctx):
+    """Points on vertices and edge midpoints (ties between triangles), just off
+    the surface and far away: the bounding-sphere culling must leave the
+    binary64 distance identical to the full search of the oracle."""
+    from its_amd import _capi
+    c = case("bunny", 32)
+    rng = np.random.default_rng(7)
+    V, F = c.V, c.F
+    mids = 0.5 * (V[F[:, 0]] + V[F[:, 1]])
+    near = V[rng.choice(len(V), 500)] + 1e-4 * rng.standard_normal((500, 3))
+    far = rng.uniform(-3, 4, size=(500, 3))
+    P = np.concatenate([V[:500], mids[:500], near, far])
+    d = ctx.signed_distance(V, F, P)
+    ref = O.signed_distance(V, F, P)
+    # the culling decides which triangles get the binary64 distance: exact
+    assert np.abs(np.abs(d) - np.abs(ref)).max() <= 1e-12
+    # the sign comes from the binary32 winding number, unchanged by the
+    # culling; on the surface and where the open mesh leaves w near 1/2 it is
+    # a tie, so compare it only where the binary64 winding number is clear
+    w = np.array([O.winding_number(V, F, p) for p in P])
+    clear = (np.abs(ref) > 1e-6) & (np.abs(w - 0.5) > 0.05)
+    assert clear.sum() > 900
+    np.testing.assert_array_equal(np.sign(d[clear]), np.sign(ref[clear]))

@@ -63,8 +63,7 @@ def test_demo_matches_python_path(ctx, tmp_path):
     svo = _capi.Octree(ctx, meshes.triangles(Vn, F), origin, width, R)
     b = svo.sdf_at_vertices(Vn, F)
     lam, st = svo.lscg(b, tol=1e-6)
-    f = svo.eval(Vn, end, _capi.EVAL_SUPPORT, 0.0)
-    inner, outer = float(f.min()), float(f.max())
+    inner, outer, _ = svo.shell_iso(Vn, F)
     q = _queries(origin, width, nq)
     fq = svo.eval(q, end, _capi.EVAL_SUPPORT, np.float32(outer))
     cls = _capi.classify(fq, inner, outer)
