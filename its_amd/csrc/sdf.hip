@@ -2,29 +2,50 @@
 //
 // Replaces ThinShells::cpSDFOfTreeNodes (reference ThinShells.cpp:481-509).
 // The reference calls igl::signed_distance with the fast winding number sign,
-// which is third-party code absent here.  Both this kernel and the CPU oracle
+// which is third-party code absent here.  Both this file and the CPU oracle
 // use the same definition:
 //   - |b| is the exact Euclidean distance to the nearest triangle (closest
 //     point on a triangle, Ericson, Real-Time Collision Detection 5.1.5),
 //     in binary64.
 //   - The sign is negative where the generalised winding number exceeds 1/2.
 //     The winding number is the sum of triangle solid angles over 4 pi (Van
-//     Oosterom and Strackee 1983), accumulated in binary32.  Its only use is
-//     the comparison with 1/2, and away from the surface the winding number is
-//     within 1e-4 of 0 or 1.
-// Brute force over triangles: each 256-thread block stages 256 triangles at
-// a time in LDS, and every thread tests its point against the staged tile.
-// The winding number needs every triangle; the binary64 closest point is
-// computed only for triangles whose bounding sphere can beat the distance
-// found so far (see k_sdf), which leaves the result unchanged.
+//     Oosterom and Strackee 1983), each solid angle in binary32 after a
+//     binary64 subtraction of the query point.
+//
+// Triangle tree.  Triangles are sorted by the Morton code of their centroid
+// and cut into buckets of kBucket; an implicit binary tree sits over the
+// buckets (node k of level h covers sorted triangles [k*8*2^h, (k+1)*8*2^h)),
+// with a binary32 box per node rounded outward, so it holds every vertex
+// exactly.  Levels are built one launch each, without atomics.
+//   - Distance: depth-first walk that skips a node whose box is farther than
+//     the best binary64 distance so far, started from the bucket at the
+//     query's own Morton position.  A skipped node holds no triangle whose
+//     distance could equal or beat the best, so the minimum and the lowest
+//     index among equal minima are those of the full search.
+//   - Winding number, exactly (generalised winding numbers, Jacobson,
+//     Kavan and Sorkine-Hornung 2013, section 3.2): for a node with triangle
+//     set S and a query p outside the node's box, close S with the fan C of
+//     its boundary edges around the box centre; S + C is a closed chain
+//     inside the convex box, so its winding number at p is 0 and
+//     w_S(p) = -w_C(p).  The boundary of S is the signed sum of its half
+//     edges (a -> b counts +1 for the min -> max direction, -1 otherwise),
+//     which is exact for open, non-manifold or inconsistently oriented
+//     meshes too.  A node's fan is stored only when it has at most as many
+//     edges as the node has triangles; otherwise the walk descends.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include <algorithm>
+#include <cmath>
 
 #include "internal.hpp"
 
 namespace its {
 namespace {
 constexpr int kT = 256;
+constexpr int kBucket = 8;
+constexpr int kMaxLevels = 32;
 
+// ------------------------------------------------------------ geometry ----
 __device__ inline double d3dot(const double* a, const double* b) { return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]); }
 
 __device__ double point_tri_dist2(const double p[3], const double a[3], const double b[3], const double c[3]) {
@@ -64,33 +85,68 @@ __device__ double point_tri_dist2(const double p[3], const double a[3], const do
     return d3dot(d, d);
 }
 
-__device__ inline float solid_angle_f(const float p[3], const double* A, const double* B, const double* C) {
+// atan2 for the solid-angle sums: octant reduction to a = min/max in
+// [0, 1] and atan(a) = a P(a^2), P of degree 7 fitted here by weighted least
+// squares on Chebyshev nodes (Lawson iterations toward the minimax error).
+// Absolute error below 3.2e-7 over all quadrants in binary32, signed zeros
+// and the +-pi cut as atan2.
+__device__ inline float atan2_fit(float y, float x) {
+#pragma clang fp contract(fast)
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 1e-30f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+    const float q = a * a;
+    float r = -0.004054564982652664f;
+    r = r * q + 0.021862955763936043f;
+    r = r * q - 0.055912334471940994f;
+    r = r * q + 0.0964219868183136f;
+    r = r * q - 0.1390863060951233f;
+    r = r * q + 0.19946566224098206f;
+    r = r * q - 0.33329859375953674f;
+    r = r * q + 0.9999993443489075f;
+    r = r * a;
+    if (ay > ax) r = 1.5707963267948966f - r;
+    if (x < 0.f) r = 3.141592653589793f - r;
+    return copysignf(r, y);
+}
+
+// solid angle of triangle ABC seen from p (signed by orientation): binary64
+// differences, the rest in binary32 with fused multiply-adds
+__device__ inline float solid_angle(const double p[3], const double* A, const double* B, const double* C) {
+#pragma clang fp contract(fast)
     float a[3], b[3], c[3];
     for (int i = 0; i < 3; ++i) {
-        a[i] = (float)(A[i] - (double)p[i]);
-        b[i] = (float)(B[i] - (double)p[i]);
-        c[i] = (float)(C[i] - (double)p[i]);
+        a[i] = (float)(A[i] - p[i]);
+        b[i] = (float)(B[i] - p[i]);
+        c[i] = (float)(C[i] - p[i]);
     }
-    float la = sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    float lb = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    float lc = sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    float la = __builtin_amdgcn_sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    float lb = __builtin_amdgcn_sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    float lc = __builtin_amdgcn_sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
     float det = a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
     float ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
     float bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
     float ca = c[0] * a[0] + c[1] * a[1] + c[2] * a[2];
     float den = la * lb * lc + ab * lc + bc * la + ca * lb;
-    return 2.0f * atan2f(det, den);
+    return 2.0f * atan2_fit(det, den);
 }
 
-// Per-triangle bounding sphere in binary32: centre = vertex mean, radius =
-// largest centre-vertex distance, both rounded outward by a margin that
-// covers the binary32 rounding of every quantity derived from them.
+// binary32 neighbours of a binary64 value, rounded down / up
+__device__ inline float f_down(double x) {
+    float f = (float)x;
+    return ((double)f > x) ? nextafterf(f, -INFINITY) : f;
+}
+__device__ inline float f_up(double x) {
+    float f = (float)x;
+    return ((double)f < x) ? nextafterf(f, INFINITY) : f;
+}
+
+// Bounding sphere of a triangle in binary32: centre = vertex mean, radius =
+// largest centre-vertex distance, both widened by a margin that covers the
+// binary32 rounding of every quantity derived from them.
 __device__ inline float bound_margin(float scale) { return 1e-5f * scale + 1e-30f; }
 
-__global__ void k_tri_spheres(const double* tris, int64_t nF, float4* sph) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nF) return;
-    const double* T = tris + 9 * t;
+__device__ inline float4 tri_sphere(const double* T) {
     double c[3];
     for (int a = 0; a < 3; ++a) c[a] = (T[a] + T[3 + a] + T[6 + a]) / 3.0;
     double r2 = 0, cm = 0;
@@ -104,100 +160,595 @@ __global__ void k_tri_spheres(const double* tris, int64_t nF, float4* sph) {
     }
     for (int a = 0; a < 3; ++a) cm = fmax(cm, fabs(c[a]));
     const double r = sqrt(r2);
-    sph[t] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)(r * (1.0 + 1e-6) + 1e-6 * (cm + r)));
+    return make_float4((float)c[0], (float)c[1], (float)c[2], (float)(r * (1.0 + 1e-6) + 1e-6 * (cm + r)));
 }
 
-// Nearest triangle in binary64 (Ericson's closest point) and the winding
-// number (binary32 solid angles), brute force over LDS tiles of triangles.
-// A first pass over the triangles' bounding spheres gives an upper bound on
-// the point's distance; the second pass computes the exact binary64 distance
-// only for triangles whose sphere can come closer than the running best.  A
-// triangle it skips is strictly farther than the nearest one, so the minimum
-// and the index of its first occurrence are those of the full search.
-__global__ __launch_bounds__(kT) void k_sdf(const double* tris, const float4* sph, int64_t nF, const double* pts,
-                                           int64_t N, double* out, int32_t* tri_out) {
-    __shared__ double sh[kT * 9];
-    __shared__ float4 ss[kT];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double p[3] = {0, 0, 0};
-    if (i < N) { p[0] = pts[3 * i]; p[1] = pts[3 * i + 1]; p[2] = pts[3 * i + 2]; }
-    float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
-    const float pm = fmaxf(fabsf(pf[0]), fmaxf(fabsf(pf[1]), fabsf(pf[2])));
-    // pass 1: upper bound of the distance from the bounding spheres
-    float ub = 3.0e38f;
-    for (int64_t base = 0; base < nF; base += kT) {
-        __syncthreads();
-        int64_t t = base + threadIdx.x;
-        if (t < nF) ss[threadIdx.x] = sph[t];
-        __syncthreads();
-        const int cnt = (int)std::min<int64_t>(kT, nF - base);
-        if (i < N)
-            for (int k = 0; k < cnt; ++k) {
-                const float4 S = ss[k];
-                const float dx = pf[0] - S.x, dy = pf[1] - S.y, dz = pf[2] - S.z;
-                const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-                const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
-                ub = fminf(ub, d + S.w + m);
-            }
-    }
-    double best = (ub < 3.0e38f) ? (double)ub * (double)ub * (1.0 + 1e-6) : 1e300;
-    int64_t best_t = -1;
-    float wsum = 0.f;
-    // pass 2: exact distance where it can matter, and the winding number
-    for (int64_t base = 0; base < nF; base += kT) {
-        __syncthreads();
-        int64_t t = base + threadIdx.x;
-        if (t < nF) {
-            for (int k = 0; k < 9; ++k) sh[threadIdx.x * 9 + k] = tris[9 * t + k];
-            ss[threadIdx.x] = sph[t];
-        }
-        __syncthreads();
-        int cnt = (int)std::min<int64_t>(kT, nF - base);
-        if (i < N) {
-            for (int k = 0; k < cnt; ++k) {
-                const double* T = &sh[9 * k];
-                const float4 S = ss[k];
-                const float dx = pf[0] - S.x, dy = pf[1] - S.y, dz = pf[2] - S.z;
-                const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
-                const float lb = sqrtf(dx * dx + dy * dy + dz * dz) - S.w - m;
-                if (!(lb > 0.f && (double)lb * (double)lb > best)) {
-                    const double d2 = point_tri_dist2(p, T, T + 3, T + 6);
-                    if (d2 < best) { best = d2; best_t = base + k; }
-                }
-                wsum += solid_angle_f(pf, T, T + 3, T + 6);
-            }
-        }
-    }
-    if (i < N) {
-        double d = sqrt(best);
-        float w = wsum * (float)(0.25 / M_PI);
-        out[i] = (w > 0.5f) ? -d : d;
-        if (tri_out) tri_out[i] = (int32_t)best_t;
-    }
+// ---------------------------------------------------------------- tree ----
+struct Shape {
+    int32_t T;                  // triangles
+    int32_t H;                  // root level; level 0 holds the buckets
+    int32_t count[kMaxLevels];  // nodes per level
+    int32_t off[kMaxLevels];    // first node of each level
+};
+
+__host__ __device__ inline int64_t node_first(int h, int64_t k) { return k * ((int64_t)kBucket << h); }
+__host__ __device__ inline int64_t node_last(const Shape& s, int h, int64_t k) {
+    return std::min<int64_t>(s.T, (k + 1) * ((int64_t)kBucket << h));
 }
 
-__global__ void k_gather_tris(const double* V, const int32_t* F, int64_t nF, double* tris) {
+// next node in depth-first order after the subtree at (h, k); false at the end
+__device__ inline bool tree_next(const Shape& s, int& h, int& k) {
+    while (h < s.H) {
+        if (!(k & 1) && k + 1 < s.count[h]) {
+            ++k;
+            return true;
+        }
+        k >>= 1;
+        ++h;
+    }
+    return false;
+}
+
+struct Quant {
+    double lo[3], scale[3];
+};
+
+__device__ inline uint32_t quant_code(const Quant& q, const double x[3]) {
+    uint32_t u[3];
+    for (int a = 0; a < 3; ++a) {
+        double v = (x[a] - q.lo[a]) * q.scale[a];
+        v = fmin(fmax(v, 0.0), 1023.0);
+        u[a] = (uint32_t)v;
+    }
+    return morton3(u[0], u[1], u[2]);
+}
+
+struct TreeView {
+    Shape s;
+    Quant q;
+    const double* tris;      // 9 per sorted triangle
+    const float4* sph;       // bounding sphere per sorted triangle
+    const int32_t* tid;      // original index per sorted triangle
+    const uint32_t* code;    // centroid Morton code per sorted triangle (ascending)
+    const float4* node;      // 2 per node: box lo.xyz + first fan edge (int bits in w),
+                             // box hi.xyz + fan edge count (int bits in w; -1: no fan)
+    const double* fan;       // 6 per fan edge: a xyz, b xyz
+};
+
+// -------------------------------------------------------------- build -----
+__global__ void k_centroid_box(const double* V, const int32_t* F, int64_t nF, double* part) {
+    __shared__ double sh[6][kT];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nF; t += (int64_t)gridDim.x * blockDim.x) {
+        const double* a = V + 3 * (int64_t)F[3 * t];
+        const double* b = V + 3 * (int64_t)F[3 * t + 1];
+        const double* c = V + 3 * (int64_t)F[3 * t + 2];
+        for (int x = 0; x < 3; ++x) {
+            const double m = (a[x] + b[x] + c[x]) / 3.0;
+            lo[x] = fmin(lo[x], m);
+            hi[x] = fmax(hi[x], m);
+        }
+    }
+    for (int x = 0; x < 3; ++x) { sh[x][threadIdx.x] = lo[x]; sh[3 + x][threadIdx.x] = hi[x]; }
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int x = 0; x < 3; ++x) {
+                sh[x][threadIdx.x] = fmin(sh[x][threadIdx.x], sh[x][threadIdx.x + o]);
+                sh[3 + x][threadIdx.x] = fmax(sh[3 + x][threadIdx.x], sh[3 + x][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_tri_keys(const double* V, const int32_t* F, int64_t nF, Quant q, int tb, uint64_t* keys) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nF) return;
+    const double* a = V + 3 * (int64_t)F[3 * t];
+    const double* b = V + 3 * (int64_t)F[3 * t + 1];
+    const double* c = V + 3 * (int64_t)F[3 * t + 2];
+    double m[3];
+    for (int x = 0; x < 3; ++x) m[x] = (a[x] + b[x] + c[x]) / 3.0;
+    keys[t] = ((uint64_t)quant_code(q, m) << tb) | (uint64_t)t;
+}
+
+__global__ void k_tri_gather(const uint64_t* keys, int tb, int64_t nF, const double* V, const int32_t* F, double* tris,
+                             float4* sph, int32_t* tid, uint32_t* code, int32_t* pos) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nF) return;
+    const uint64_t k = keys[i];
+    const int64_t t = (int64_t)(k & ((1ull << tb) - 1));
+    double T[9];
     for (int c = 0; c < 3; ++c)
-        for (int a = 0; a < 3; ++a) tris[9 * t + 3 * c + a] = V[3 * (int64_t)F[3 * t + c] + a];
+        for (int a = 0; a < 3; ++a) T[3 * c + a] = V[3 * (int64_t)F[3 * t + c] + a];
+    for (int j = 0; j < 9; ++j) tris[9 * i + j] = T[j];
+    sph[i] = tri_sphere(T);
+    tid[i] = (int32_t)t;
+    code[i] = (uint32_t)(k >> tb);
+    pos[t] = (int32_t)i;
+}
+
+__global__ void k_bucket_box(const double* tris, Shape s, float4* node) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.count[0]) return;
+    double l[3] = {INFINITY, INFINITY, INFINITY}, u[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = node_first(0, k); i < node_last(s, 0, k); ++i)
+        for (int v = 0; v < 3; ++v)
+            for (int a = 0; a < 3; ++a) {
+                l[a] = fmin(l[a], tris[9 * i + 3 * v + a]);
+                u[a] = fmax(u[a], tris[9 * i + 3 * v + a]);
+            }
+    node[2 * (s.off[0] + k)] = make_float4(f_down(l[0]), f_down(l[1]), f_down(l[2]), 0.f);
+    node[2 * (s.off[0] + k) + 1] = make_float4(f_up(u[0]), f_up(u[1]), f_up(u[2]), 0.f);
+}
+
+__global__ void k_level_box(Shape s, int h, float4* node) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.count[h]) return;
+    const int64_t c0 = s.off[h - 1] + 2 * k;
+    float4 l = node[2 * c0], u = node[2 * c0 + 1];
+    if (2 * k + 1 < s.count[h - 1]) {
+        const float4 l1 = node[2 * c0 + 2], u1 = node[2 * c0 + 3];
+        l = make_float4(fminf(l.x, l1.x), fminf(l.y, l1.y), fminf(l.z, l1.z), 0.f);
+        u = make_float4(fmaxf(u.x, u1.x), fmaxf(u.y, u1.y), fmaxf(u.z, u1.z), 0.f);
+    }
+    node[2 * (s.off[h] + k)] = l;
+    node[2 * (s.off[h] + k) + 1] = u;
+}
+
+// Half edges keyed by their undirected edge (min << vb | max); the value is
+// the sorted position of the triangle times 2, plus 1 for a max -> min edge.
+// Degenerate edges (a == b) close nothing and get the key that sorts last.
+__global__ void k_half_edges(const int32_t* F, int64_t nF, const int32_t* pos, int vb, uint64_t none, uint64_t* keys,
+                             uint32_t* vals) {
+    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= 3 * nF) return;
+    const int64_t t = h / 3;
+    const int e = (int)(h - 3 * t);
+    const uint32_t a = (uint32_t)F[3 * t + e], b = (uint32_t)F[3 * t + (e + 1) % 3];
+    keys[h] = (a == b) ? none : (((uint64_t)std::min(a, b) << vb) | std::max(a, b));
+    vals[h] = 2u * (uint32_t)pos[t] + (a > b ? 1u : 0u);
+}
+
+// Walks the boundary contributions of one undirected edge (the run of equal
+// keys starting at head i): for every level h >= 1 and every node holding
+// some of its half edges, coef = signed count of those half edges.  Calls
+// sink(node, h, k, coef) for coef != 0.  Above the level where one node
+// holds all half edges, coef stays at their total; the walk stops there when
+// the total is 0 (an interior edge).
+template <class Sink>
+__device__ void edge_walk(const uint64_t* keys, const uint32_t* vals, int64_t i, int64_t M, const Shape& s, Sink&& sink) {
+    int64_t end = i + 1;
+    while (end < M && keys[end] == keys[i]) ++end;
+    for (int h = 1; h <= s.H; ++h) {
+        const int64_t span = (int64_t)kBucket << h;
+        const int64_t k0 = (int64_t)(vals[i] >> 1) / span;
+        bool same = true;
+        int tot = 0;
+        for (int64_t j = i; j < end; ++j) {
+            const uint32_t v = vals[j];
+            if ((int64_t)(v >> 1) / span != k0) same = false;
+            tot += (v & 1u) ? -1 : 1;
+        }
+        if (same) {
+            if (tot == 0) return;
+            sink(s.off[h] + k0, h, k0, tot);
+            continue;
+        }
+        for (int64_t j = i; j < end; ++j) {
+            const int64_t kj = (int64_t)(vals[j] >> 1) / span;
+            bool seen = false;
+            for (int64_t jj = i; jj < j && !seen; ++jj) seen = ((int64_t)(vals[jj] >> 1) / span == kj);
+            if (seen) continue;
+            int coef = 0;
+            for (int64_t jj = j; jj < end; ++jj)
+                if ((int64_t)(vals[jj] >> 1) / span == kj) coef += (vals[jj] & 1u) ? -1 : 1;
+            if (coef) sink(s.off[h] + kj, h, kj, coef);
+        }
+    }
+}
+
+__device__ inline bool edge_head(const uint64_t* keys, int64_t i, uint64_t none) {
+    return keys[i] != none && (i == 0 || keys[i] != keys[i - 1]);
+}
+
+__global__ void k_fan_node_count(const uint64_t* keys, const uint32_t* vals, int64_t M, Shape s, uint64_t none,
+                                 uint32_t* node_cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M || !edge_head(keys, i, none)) return;
+    edge_walk(keys, vals, i, M, s, [&](int64_t node, int, int64_t, int coef) {
+        atomicAdd(&node_cnt[node], (uint32_t)abs(coef));
+    });
+}
+
+__device__ inline bool fan_kept(const Shape& s, const uint32_t* node_cnt, int64_t node, int h, int64_t k) {
+    return (int64_t)node_cnt[node] <= node_last(s, h, k) - node_first(h, k);
+}
+
+__global__ void k_fan_edge_count(const uint64_t* keys, const uint32_t* vals, int64_t M, Shape s, uint64_t none,
+                                 const uint32_t* node_cnt, uint32_t* cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    uint32_t c = 0;
+    if (edge_head(keys, i, none))
+        edge_walk(keys, vals, i, M, s, [&](int64_t node, int h, int64_t k, int coef) {
+            if (fan_kept(s, node_cnt, node, h, k)) c += (uint32_t)abs(coef);
+        });
+    cnt[i] = c;
+}
+
+__global__ void k_fan_emit(const uint64_t* keys, const uint32_t* vals, int64_t M, Shape s, uint64_t none, int vb,
+                           const uint32_t* node_cnt, const uint32_t* off, uint32_t* fan_node, uint64_t* fan_edge) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M || !edge_head(keys, i, none)) return;
+    const uint64_t lo = keys[i] >> vb, hi = keys[i] & ((1ull << vb) - 1);
+    uint32_t o = off[i];
+    edge_walk(keys, vals, i, M, s, [&](int64_t node, int h, int64_t k, int coef) {
+        if (!fan_kept(s, node_cnt, node, h, k)) return;
+        const uint64_t e = coef > 0 ? ((lo << 32) | hi) : ((hi << 32) | lo);
+        for (int r = 0; r < abs(coef); ++r) {
+            fan_node[o] = (uint32_t)node;
+            fan_edge[o] = e;
+            ++o;
+        }
+    });
+}
+
+__global__ void k_fan_table(Shape s, int64_t nodes, const uint32_t* node_cnt, const uint32_t* fan_off, float4* node) {
+    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    int h = 0;
+    while (h < s.H && n >= s.off[h + 1]) ++h;
+    const int64_t k = n - s.off[h];
+    const bool kept = h > 0 && fan_kept(s, node_cnt, n, h, k);
+    node[2 * n].w = __int_as_float((int)fan_off[n]);
+    node[2 * n + 1].w = __int_as_float(kept ? (int)node_cnt[n] : -1);
+}
+
+__global__ void k_fan_coords(const uint64_t* fan_edge, int64_t E, const double* V, double* fan) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const uint64_t ab = fan_edge[e];
+    const int64_t a = (int64_t)(ab >> 32), b = (int64_t)(ab & 0xffffffffull);
+    for (int x = 0; x < 3; ++x) {
+        fan[6 * e + x] = V[3 * a + x];
+        fan[6 * e + 3 + x] = V[3 * b + x];
+    }
+}
+
+// kept fan edges per node (0 for dropped fans and buckets), for the scan
+__global__ void k_fan_kept_count(Shape s, int64_t nodes, const uint32_t* node_cnt, uint32_t* kept) {
+    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    int h = 0;
+    while (h < s.H && n >= s.off[h + 1]) ++h;
+    const int64_t k = n - s.off[h];
+    kept[n] = (h > 0 && fan_kept(s, node_cnt, n, h, k)) ? node_cnt[n] : 0u;
+}
+
+int bits_for(uint64_t maxval) {
+    int b = 1;
+    while (b < 64 && (maxval >> b) != 0) ++b;
+    return b;
+}
+
+template <class K, class Vv>
+void radix_sort_pairs(DBuf<K>& keys, DBuf<Vv>& vals, int64_t n, int bits, hipStream_t st) {
+    DBuf<K> k2(std::max<int64_t>(n, 1));
+    DBuf<Vv> v2(std::max<int64_t>(n, 1));
+    size_t tmp = 0;
+    ITS_HIP(rocprim::radix_sort_pairs(nullptr, tmp, keys.p, k2.p, vals.p, v2.p, (size_t)n, 0, bits, st));
+    DBuf<char> t(std::max<size_t>(tmp, 1));
+    ITS_HIP(rocprim::radix_sort_pairs(t.p, tmp, keys.p, k2.p, vals.p, v2.p, (size_t)n, 0, bits, st));
+    std::swap(keys, k2);
+    std::swap(vals, v2);
+}
+
+void radix_sort_keys(DBuf<uint64_t>& keys, int64_t n, int bits, hipStream_t st) {
+    DBuf<uint64_t> k2(std::max<int64_t>(n, 1));
+    size_t tmp = 0;
+    ITS_HIP(rocprim::radix_sort_keys(nullptr, tmp, keys.p, k2.p, (size_t)n, 0, bits, st));
+    DBuf<char> t(std::max<size_t>(tmp, 1));
+    ITS_HIP(rocprim::radix_sort_keys(t.p, tmp, keys.p, k2.p, (size_t)n, 0, bits, st));
+    std::swap(keys, k2);
+}
+
+struct TriTree {
+    Shape s{};
+    Quant q{};
+    int64_t nodes = 0, fans = 0;
+    DBuf<double> tris, fan;
+    DBuf<float4> sph, node;
+    DBuf<int32_t> tid;
+    DBuf<uint32_t> code;
+    TreeView view() const {
+        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p};
+    }
+};
+
+void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, const int32_t* d_F, int64_t nF) {
+    ITS_REQUIRE(nF > 0 && nF < (1ll << 30) && nV < (1ll << 31), ITS_E_RANGE, "mesh too large for the triangle tree (2^30 triangles, 2^31 vertices)");
+    Shape& s = tr.s;
+    s.T = (int32_t)nF;
+    s.count[0] = (int32_t)((nF + kBucket - 1) / kBucket);
+    s.off[0] = 0;
+    s.H = 0;
+    while (s.count[s.H] > 1) {
+        ITS_REQUIRE(s.H + 1 < kMaxLevels, ITS_E_INVALID, "triangle tree too deep");
+        s.count[s.H + 1] = (s.count[s.H] + 1) / 2;
+        s.off[s.H + 1] = s.off[s.H] + s.count[s.H];
+        ++s.H;
+    }
+    tr.nodes = (int64_t)s.off[s.H] + 1;
+
+    // Morton quantisation over the centroid box
+    const int nb = (int)std::min<int64_t>(256, blocks_for(nF, kT));
+    DBuf<double> part(6 * (size_t)nb);
+    hipLaunchKernelGGL(k_centroid_box, dim3(nb), dim3(kT), 0, st, d_V, d_F, nF, part.p);
+    ITS_LAUNCH_CHECK();
+    std::vector<double> hp;
+    to_host(hp, part.p, 6 * (size_t)nb, st);
+    ITS_HIP(hipStreamSynchronize(st));
+    for (int a = 0; a < 3; ++a) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int b = 0; b < nb; ++b) { lo = std::min(lo, hp[6 * b + a]); hi = std::max(hi, hp[6 * b + 3 + a]); }
+        tr.q.lo[a] = lo;
+        tr.q.scale[a] = (hi > lo) ? 1024.0 / (hi - lo) : 0.0;
+    }
+
+    // sort triangles by centroid code (ties by index)
+    const int tb = bits_for((uint64_t)(nF - 1));
+    DBuf<uint64_t> tkeys(nF);
+    hipLaunchKernelGGL(k_tri_keys, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, d_V, d_F, nF, tr.q, tb, tkeys.p);
+    ITS_LAUNCH_CHECK();
+    radix_sort_keys(tkeys, nF, 30 + tb, st);
+    tr.tris.alloc(9 * (size_t)nF);
+    tr.sph.alloc(nF);
+    tr.tid.alloc(nF);
+    tr.code.alloc(nF);
+    DBuf<int32_t> pos(nF);
+    hipLaunchKernelGGL(k_tri_gather, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, tkeys.p, tb, nF, d_V, d_F, tr.tris.p,
+                       tr.sph.p, tr.tid.p, tr.code.p, pos.p);
+    ITS_LAUNCH_CHECK();
+
+    // node boxes, one level per launch
+    tr.node.alloc(2 * tr.nodes);
+    hipLaunchKernelGGL(k_bucket_box, dim3(blocks_for(s.count[0], kT)), dim3(kT), 0, st, tr.tris.p, s, tr.node.p);
+    ITS_LAUNCH_CHECK();
+    for (int h = 1; h <= s.H; ++h) {
+        hipLaunchKernelGGL(k_level_box, dim3(blocks_for(s.count[h], kT)), dim3(kT), 0, st, s, h, tr.node.p);
+        ITS_LAUNCH_CHECK();
+    }
+
+    // boundary fans: half edges grouped by undirected edge
+    const int64_t M = 3 * nF;
+    const int vb = bits_for((uint64_t)(nV - 1));
+    const uint64_t none = (2 * vb >= 64) ? ~0ull : ((1ull << (2 * vb)) - 1);
+    DBuf<uint64_t> hkeys(M);
+    DBuf<uint32_t> hvals(M);
+    hipLaunchKernelGGL(k_half_edges, dim3(blocks_for(M, kT)), dim3(kT), 0, st, d_F, nF, (const int32_t*)pos.p, vb, none,
+                       hkeys.p, hvals.p);
+    ITS_LAUNCH_CHECK();
+    radix_sort_pairs(hkeys, hvals, M, std::min(64, 2 * vb), st);
+
+    DBuf<uint32_t> node_cnt(tr.nodes);
+    ITS_HIP(hipMemsetAsync(node_cnt.p, 0, tr.nodes * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_fan_node_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
+                       node_cnt.p);
+    ITS_LAUNCH_CHECK();
+    DBuf<uint32_t> ecnt(M);
+    hipLaunchKernelGGL(k_fan_edge_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
+                       (const uint32_t*)node_cnt.p, ecnt.p);
+    ITS_LAUNCH_CHECK();
+    const uint64_t E = scan_exclusive_u32(ecnt.p, ecnt.p, M, st);
+    ITS_REQUIRE(E < (1ull << 31), ITS_E_RANGE, "triangle tree: more than 2^31 boundary fan edges");
+    tr.fans = (int64_t)E;
+    DBuf<uint32_t> fan_node(std::max<uint64_t>(E, 1));
+    DBuf<uint64_t> fan_edge(std::max<uint64_t>(E, 1));
+    if (E) {
+        hipLaunchKernelGGL(k_fan_emit, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none, vb,
+                           (const uint32_t*)node_cnt.p, (const uint32_t*)ecnt.p, fan_node.p, fan_edge.p);
+        ITS_LAUNCH_CHECK();
+        // stable: within a node, fan edges stay in edge-key order
+        radix_sort_pairs(fan_node, fan_edge, (int64_t)E, bits_for((uint64_t)tr.nodes), st);
+    }
+    DBuf<uint32_t> fan_off(tr.nodes);
+    hipLaunchKernelGGL(k_fan_kept_count, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, s, tr.nodes,
+                       (const uint32_t*)node_cnt.p, fan_off.p);
+    ITS_LAUNCH_CHECK();
+    const uint64_t E2 = scan_exclusive_u32(fan_off.p, fan_off.p, tr.nodes, st);
+    ITS_REQUIRE(E2 == E, ITS_E_STATE, "triangle tree: fan edge counts disagree");
+    hipLaunchKernelGGL(k_fan_table, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, s, tr.nodes,
+                       (const uint32_t*)node_cnt.p, (const uint32_t*)fan_off.p, tr.node.p);
+    ITS_LAUNCH_CHECK();
+    tr.fan.alloc(6 * std::max<uint64_t>(E, 1));
+    if (E) {
+        hipLaunchKernelGGL(k_fan_coords, dim3(blocks_for((int64_t)E, kT)), dim3(kT), 0, st, fan_edge.p, (int64_t)E,
+                           d_V, tr.fan.p);
+        ITS_LAUNCH_CHECK();
+    }
+    ITS_HIP(hipStreamSynchronize(st));
+}
+
+// -------------------------------------------------------------- query -----
+// Both walks run one wavefront per point: the walk is the same for all
+// lanes, and nodes of level kLeafLevel and below (at most 64 triangles) are
+// leaves whose triangles the 64 lanes take one each.  Per-lane results are
+// combined by a fixed butterfly, so nothing depends on timing.
+constexpr int kLeafLevel = 3;
+static_assert((kBucket << kLeafLevel) == 64, "one triangle per lane in a leaf");
+constexpr int kWavesPerBlock = kT / 64;
+
+struct NodeRec {
+    float4 lo, hi;
+    int first, count;
+};
+__device__ inline NodeRec load_node(const TreeView& t, int node) {
+    const float4 lo = t.node[2 * node], hi = t.node[2 * node + 1];
+    return NodeRec{lo, hi, __float_as_int(lo.w), __float_as_int(hi.w)};
+}
+
+// minimum of (d2, index) over the wavefront, ties to the lower index
+__device__ inline void wave_argmin(double& d2, int32_t& id) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double od = __shfl_xor(d2, o, 64);
+        const int32_t oi = __shfl_xor(id, o, 64);
+        if (od < d2 || (od == d2 && oi < id)) {
+            d2 = od;
+            id = oi;
+        }
+    }
+}
+
+// nearest triangle of a leaf, one triangle per lane; a triangle whose
+// bounding sphere is farther than the best so far is skipped
+__device__ inline void leaf_nearest(const TreeView& t, int h, int64_t k, int lane, const double p[3], float pm,
+                                    double& best, int32_t& bt) {
+    const int64_t i = node_first(h, k) + lane;
+    double d2 = INFINITY;
+    int32_t id = 0x7fffffff;
+    if (i < node_last(t.s, h, k)) {
+        const float4 S = t.sph[i];
+        const float dx = (float)p[0] - S.x, dy = (float)p[1] - S.y, dz = (float)p[2] - S.z;
+        const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
+        const float lb = sqrtf(dx * dx + dy * dy + dz * dz) - S.w - m;
+        if (!(lb > 0.f && (double)lb * (double)lb > best)) {
+            const double* T = t.tris + 9 * i;
+            d2 = point_tri_dist2(p, T, T + 3, T + 6);
+            id = t.tid[i];
+            // a degenerate triangle can give 0/0; like a plain "d2 < best"
+            // scan, a NaN never wins (and must not reach the butterfly)
+            if (!(d2 <= INFINITY)) {
+                d2 = INFINITY;
+                id = 0x7fffffff;
+            }
+        }
+    }
+    wave_argmin(d2, id);
+    if (d2 < best || (d2 == best && id < bt)) {
+        best = d2;
+        bt = id;
+    }
+}
+
+// true when no point of the box can be within the best distance found so
+// far: binary32 distance to the box (boxes are rounded outward, so they hold
+// every vertex), reduced by a margin that covers its rounding
+__device__ inline bool box_beyond(const float pf[3], float pm, float4 lo, float4 hi, double best) {
+    const float l[3] = {lo.x, lo.y, lo.z}, u[3] = {hi.x, hi.y, hi.z};
+    float d2 = 0.f, sc = pm;
+    for (int a = 0; a < 3; ++a) {
+        const float e = fmaxf(fmaxf(l[a] - pf[a], pf[a] - u[a]), 0.f);
+        d2 += e * e;
+        sc = fmaxf(sc, fmaxf(fabsf(l[a]), fabsf(u[a])));
+    }
+    const float lb = sqrtf(d2) - bound_margin(sc);
+    return lb > 0.f && (double)lb * (double)lb > best;
+}
+
+__device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
+    return p[0] < (double)lo.x || p[0] > (double)hi.x || p[1] < (double)lo.y || p[1] > (double)hi.y ||
+           p[2] < (double)lo.z || p[2] > (double)hi.z;
+}
+
+// Nearest triangle: binary64 distance (unsigned, into out) and the original
+// index of the nearest triangle.  The walk starts with the leaf at the
+// point's Morton position, which usually holds the nearest triangle.
+__global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, int64_t N, double* out,
+                                                 int32_t* tri_out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+    const int leaf = std::min(kLeafLevel, (int)t.s.H);
+    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); i < N; i += nw) {
+        const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+        const float pm = fmaxf(fabsf(pf[0]), fmaxf(fabsf(pf[1]), fabsf(pf[2])));
+        double best = INFINITY;
+        int32_t bt = 0x7fffffff;
+        const uint32_t c = quant_code(t.q, p);
+        int64_t a = 0, b = t.s.T;
+        while (a < b) {
+            const int64_t m = (a + b) >> 1;
+            if (t.code[m] < c) a = m + 1; else b = m;
+        }
+        const int64_t seed = std::min<int64_t>(a, t.s.T - 1) / ((int64_t)kBucket << leaf);
+        leaf_nearest(t, leaf, seed, lane, p, pm, best, bt);
+        int h = t.s.H, k = 0;
+        while (true) {
+            const NodeRec r = load_node(t, t.s.off[h] + k);
+            if (!(h == leaf && k == seed) && !box_beyond(pf, pm, r.lo, r.hi, best)) {
+                if (h > leaf) {
+                    --h;
+                    k <<= 1;
+                    continue;
+                }
+                leaf_nearest(t, h, k, lane, p, pm, best, bt);
+            }
+            if (!tree_next(t.s, h, k)) break;
+        }
+        if (lane == 0) {
+            out[i] = sqrt(best);
+            if (tri_out) tri_out[i] = bt;
+        }
+    }
+}
+
+// Winding number: fans of the nodes the point is outside of, triangles of
+// the leaves it reaches.  Applies the sign to the distance in out.
+__global__ __launch_bounds__(kT) void k_sdf_wind(TreeView t, const double* pts, int64_t N, double* out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+    const int leaf = std::min(kLeafLevel, (int)t.s.H);
+    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); i < N; i += nw) {
+        const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        float acc = 0.f;
+        int h = t.s.H, k = 0;
+        while (true) {
+            const NodeRec r = load_node(t, t.s.off[h] + k);
+            if (h <= leaf) {
+                const int64_t j = node_first(h, k) + lane;
+                if (j < node_last(t.s, h, k)) {
+                    const double* T = t.tris + 9 * j;
+                    acc += solid_angle(p, T, T + 3, T + 6);
+                }
+            } else if (r.count >= 0 && box_outside(p, r.lo, r.hi)) {
+                const double c[3] = {0.5 * ((double)r.lo.x + (double)r.hi.x), 0.5 * ((double)r.lo.y + (double)r.hi.y),
+                                     0.5 * ((double)r.lo.z + (double)r.hi.z)};
+                for (int e = lane; e < r.count; e += 64) {
+                    const double* ab = t.fan + 6 * ((int64_t)r.first + e);
+                    acc += solid_angle(p, c, ab, ab + 3);
+                }
+            } else {
+                --h;
+                k <<= 1;
+                continue;
+            }
+            if (!tree_next(t.s, h, k)) break;
+        }
+        const float w = wave_sum(acc) * (float)(0.25 / M_PI);
+        if (lane == 0) {
+            const double d = out[i];
+            out[i] = (w > 0.5f) ? -d : d;
+        }
+    }
 }
 }  // namespace
 
 void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_t* d_F, int64_t nF,
                      const double* d_pts, int64_t N, double* d_out, int32_t* d_tri) {
-    (void)nV;
     if (N <= 0) return;
-    DBuf<double> tris(std::max<int64_t>(9 * nF, 9));
-    DBuf<float4> sph(std::max<int64_t>(nF, 1));
-    if (nF > 0) {
-        hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, d_V, d_F, nF, tris.p);
-        ITS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_tri_spheres, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, (const double*)tris.p, nF, sph.p);
-        ITS_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_sdf, dim3(blocks_for(N, kT)), dim3(kT), 0, st, tris.p, (const float4*)sph.p, nF, d_pts, N,
-                       d_out, d_tri);
+    TriTree tr;
+    tree_build(tr, st, d_V, nV, d_F, nF);
+    // one wavefront per point, grid-strided over 8 blocks per CU
+    int dev = 0, cus = 256;
+    ITS_HIP(hipGetDevice(&dev));
+    ITS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t wblocks = std::min<int64_t>((N + kWavesPerBlock - 1) / kWavesPerBlock, 8 * (int64_t)cus);
+    hipLaunchKernelGGL(k_sdf_dist, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, d_tri);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sdf_wind, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out);
     ITS_LAUNCH_CHECK();
     ITS_HIP(hipStreamSynchronize(st));
 }
