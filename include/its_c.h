@@ -262,7 +262,7 @@ int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const
  *   (default 1), when the queue is off.
  * ITS_TUNE_EVAL_QUEUE: 1 = queries that need the fine octree levels are
  *   gathered per wavefront in LDS and walked 64 at a time; 0 = every thread
- *   walks its own query (default 0).
+ *   walks its own query (default 1).
  * ITS_TUNE_BATCH_MAX_VERTS: meshes of a batch with at most this many
  *   unknowns share the one-launch fit (default 65536).
  * No reference counterpart (the reference hard-codes its launch shapes). */

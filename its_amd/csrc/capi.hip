@@ -14,7 +14,7 @@ namespace its {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
-static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 64, 2048, 1, 0, 65536};
+static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 64, 2048, 1, 1, 65536};
 int64_t tune_get(int key) { return g_tune[key]; }
 }  // namespace its
 
@@ -530,6 +530,7 @@ int its_lattice_f64(its_ctx ctx, its_svo s, double* values) {
 }
 
 // ---- shell iso band
+extern "C++" {   // a C++ helper inside the extern "C" block: C++ linkage
 namespace {
 struct ShellInput {
     DBuf<double> V, tris;
@@ -553,6 +554,7 @@ ShellInput shell_upload(const double* V, int64_t nV, const int32_t* F, int64_t n
     return in;
 }
 }  // namespace
+}  // extern "C++"
 
 int its_shell_points(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* pts,
                      int64_t capacity, int64_t* n_edge, int64_t* n_face) {

@@ -26,7 +26,13 @@
 // as one 32-byte block of its 8 corner values, so a query reads one aligned
 // cache line for all coarse levels; lc is the finest level whose cell count
 // stays under a tunable bound (32^3 cells, 1 MB, by default: L2 resident).
-// A query inside the box then walks only the lc sparse fine levels.  The
+// A query inside the box then walks only the lc sparse fine levels.
+//
+// Child links ride in the blocks: the first marked child (32 bits) and the
+// child mask (8 bits) of a cell sit in the 5 low mantissa bits of its 8
+// corner values, so one 32-byte block per level gives both the values and
+// the way down, and a query reads one cache line per level and nothing else.
+// Scattered reads are what the walk pays for, one L1 fill each.  The
 // per-level walk over all levels remains for points outside the box (DENSE
 // and RAW modes).
 #include <algorithm>
@@ -77,13 +83,12 @@ __global__ void k_fill_blocks(int64_t nv, int64_t voff, const uint16_t* vx, cons
 struct EvalParams {
     const RankWord* cells[kMaxLevels];
     const float4* blocks[kMaxLevels];
-    const float4* G;  // per level-lc cell: 8 corner values of levels >= lc, child mask in the low bits
-    const uint32_t* Gbase;             // per level-lc cell: index of its first marked child
-    const uint32_t* cbase[kMaxLevels]; // levels 1..lc-1: index of each marked cell's first marked child
+    const float4* G;                   // per level-lc cell: 8 corner values of levels >= lc
     int L;
     int R;
     int lc;           // collapsed level
-    int Rc;           // cells per axis of G (R_lc)
+    int Rc;           // cells per axis of G (R_lc, a power of two)
+    int rc_shift;     // log2(Rc)
     float ox, oy, oz, inv_w;
     double blo[3], bhi[3];   // accept box: open (SUPPORT) or closed (DENSE)
     float flo[3], fhi[3];    // the same bounds as binary32 (see SoaF32::in_box)
@@ -112,11 +117,14 @@ struct SoaF32 {
     // For binary32 inputs the binary64 comparisons are exact in binary32
     // against bounds rounded outward on the host (flo/fhi).
     __device__ bool in_box(const EvalParams& P, float x, float y, float z, double, double, double) const {
+        // the reference's "outside" comparisons (a NaN coordinate is not
+        // outside), combined with | rather than ||: lane masks, no branches
         if (P.mode == ITS_EVAL_SUPPORT)
-            return !(x <= P.flo[0] || y <= P.flo[1] || z <= P.flo[2] || x >= P.fhi[0] || y >= P.fhi[1] ||
-                     z >= P.fhi[2]);
+            return !((x <= P.flo[0]) | (y <= P.flo[1]) | (z <= P.flo[2]) | (x >= P.fhi[0]) | (y >= P.fhi[1]) |
+                     (z >= P.fhi[2]));
         if (P.mode == ITS_EVAL_DENSE)
-            return !(x < P.flo[0] || y < P.flo[1] || z < P.flo[2] || x > P.fhi[0] || y > P.fhi[1] || z > P.fhi[2]);
+            return !((x < P.flo[0]) | (y < P.flo[1]) | (z < P.flo[2]) | (x > P.fhi[0]) | (y > P.fhi[1]) |
+                     (z > P.fhi[2]));
         return true;
     }
 };
@@ -170,6 +178,31 @@ __device__ inline float tri_clamped(float4 b0, float4 b1, float tx, float ty, fl
 
 __device__ inline float pow2neg(int d) { return __uint_as_float((uint32_t)(127 - d) << 23); }
 
+// Link payload of a block: corner value k holds mask bit k in bit 0 and bits
+// 4k..4k+3 of the first marked child in bits 1-4.  Values are rounded to a
+// multiple of 32 ulps before the payload goes in (relative error at most
+// 2^-19), and readers clear the 5 bits.
+constexpr uint32_t kLinkBits = 31u;
+__device__ inline uint32_t link_decode(float4& b0, float4& b1, uint32_t& base) {
+    uint32_t u[8] = {__float_as_uint(b0.x), __float_as_uint(b0.y), __float_as_uint(b0.z), __float_as_uint(b0.w),
+                     __float_as_uint(b1.x), __float_as_uint(b1.y), __float_as_uint(b1.z), __float_as_uint(b1.w)};
+    uint32_t mask = 0u, bs = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        mask |= (u[k] & 1u) << k;
+        bs |= ((u[k] >> 1) & 15u) << (4 * k);
+        u[k] &= ~kLinkBits;
+    }
+    b0 = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]), __uint_as_float(u[3]));
+    b1 = make_float4(__uint_as_float(u[4]), __uint_as_float(u[5]), __uint_as_float(u[6]), __uint_as_float(u[7]));
+    base = bs;
+    return mask;
+}
+__device__ inline float4 link_clear(float4 b) {
+    return make_float4(__uint_as_float(__float_as_uint(b.x) & ~kLinkBits), __uint_as_float(__float_as_uint(b.y) & ~kLinkBits),
+                       __uint_as_float(__float_as_uint(b.z) & ~kLinkBits), __uint_as_float(__float_as_uint(b.w) & ~kLinkBits));
+}
+
 // every level, root -> leaf, clamped cells and hats: any point
 __device__ float eval_walk(const EvalParams& P, float ux, float uy, float uz, int cx, int cy, int cz, uint32_t m0) {
     float f = 0.f;
@@ -180,26 +213,29 @@ __device__ float eval_walk(const EvalParams& P, float ux, float uy, float uz, in
         if (!(w.bits & bit)) break;
         const uint32_t idx = w.rank + __popc(w.bits & (bit - 1u));
         const float s = pow2neg(d);
-        f += tri_clamped(P.blocks[d][2 * idx], P.blocks[d][2 * idx + 1], ux * s - (float)(cx >> d),
-                         uy * s - (float)(cy >> d), uz * s - (float)(cz >> d));
+        float4 b0 = P.blocks[d][2 * idx], b1 = P.blocks[d][2 * idx + 1];
+        if (d > 0 && d < P.lc) {   // linked level: drop the link payload
+            b0 = link_clear(b0);
+            b1 = link_clear(b1);
+        }
+        f += tri_clamped(b0, b1, ux * s - (float)(cx >> d), uy * s - (float)(cy >> d), uz * s - (float)(cz >> d));
     }
     return f;
 }
 
-// The child mask of a cell rides in the lowest mantissa bit of its 8 corner
-// values (bit k in corner k), so one 32-byte block gives both the values and
-// the mask.  This moves each value by at most one ulp of binary32.
-__device__ inline uint32_t block_mask(float4 b0, float4 b1) {
-    return (__float_as_uint(b0.x) & 1u) | ((__float_as_uint(b0.y) & 1u) << 1) | ((__float_as_uint(b0.z) & 1u) << 2) |
-           ((__float_as_uint(b0.w) & 1u) << 3) | ((__float_as_uint(b1.x) & 1u) << 4) |
-           ((__float_as_uint(b1.y) & 1u) << 5) | ((__float_as_uint(b1.z) & 1u) << 6) |
-           ((__float_as_uint(b1.w) & 1u) << 7);
-}
+// child of the level-(d+1) cell holding voxel (cx, cy, cz): bit d of each
+// coordinate, x | y<<1 | z<<2
 __device__ inline uint32_t child_key(int cx, int cy, int cz, int d) {
-    return ((cx >> d) & 1) | (((cy >> d) & 1) << 1) | (((cz >> d) & 1) << 2);
+    return __builtin_amdgcn_ubfe((uint32_t)cx, (uint32_t)d, 1u) | (__builtin_amdgcn_ubfe((uint32_t)cy, (uint32_t)d, 1u) << 1) |
+           (__builtin_amdgcn_ubfe((uint32_t)cz, (uint32_t)d, 1u) << 2);
 }
 __device__ inline uint32_t child_index(uint32_t base, uint32_t mask, uint32_t key) {
     return base + __popc(mask & ((1u << key) - 1u));
+}
+// G cell of voxel (cx, cy, cz): x-fastest over the Rc^3 level-lc cells
+__device__ inline uint32_t g_cell(const EvalParams& P, int cx, int cy, int cz) {
+    return ((uint32_t)(cz >> P.lc) << (2 * P.rc_shift)) | ((uint32_t)(cy >> P.lc) << P.rc_shift) |
+           (uint32_t)(cx >> P.lc);
 }
 // local coordinates in the level-d cell: u in [0, R), so floor(u 2^-d) is
 // floor(u) >> d and the fraction is exact
@@ -252,53 +288,63 @@ __global__ __launch_bounds__(kT) void k_eval(Loader load, int64_t N, EvalParams 
             f[j] = ux[j] = uy[j] = uz[j] = 0.f;
             act[j] = i < N && query_front(load, P, i, f[j], ux[j], uy[j], uz[j]);
         }
-        // levels >= lc: one 32-byte G block per query
+        // levels >= lc: one 32-byte G block per query, with the child link
         float4 b0[K], b1[K];
-        uint32_t gcell[K];
+        int cx[K], cy[K], cz[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (!act[j]) continue;
-            gcell[j] = (((int)(uz[j] * sc) * P.Rc + (int)(uy[j] * sc)) * P.Rc + (int)(ux[j] * sc));
-            b0[j] = P.G[2 * (size_t)gcell[j]];
-            b1[j] = P.G[2 * (size_t)gcell[j] + 1];
+            cx[j] = (int)ux[j];   // u >= 0: truncation is floor
+            cy[j] = (int)uy[j];
+            cz[j] = (int)uz[j];
+            const uint32_t g = g_cell(P, cx[j], cy[j], cz[j]);
+            b0[j] = P.G[2 * (size_t)g];
+            b1[j] = P.G[2 * (size_t)g + 1];
         }
-        uint32_t mask[K], idx[K];
+        uint32_t idx[K];
         bool more = false;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (!act[j]) continue;
+            if (lc == 0) {
+                f[j] = tri_level(b0[j], b1[j], ux[j], uy[j], uz[j], sc);
+                act[j] = false;
+                continue;
+            }
+            uint32_t base;
+            const uint32_t mask = link_decode(b0[j], b1[j], base);
             f[j] = tri_level(b0[j], b1[j], ux[j], uy[j], uz[j], sc);
-            mask[j] = block_mask(b0[j], b1[j]);
-            const uint32_t key = lc > 0 ? child_key((int)ux[j], (int)uy[j], (int)uz[j], lc - 1) : 0u;
-            act[j] = lc > 0 && ((mask[j] >> key) & 1u);
+            const uint32_t key = child_key(cx[j], cy[j], cz[j], lc - 1);
+            act[j] = (mask >> key) & 1u;
             if (act[j]) {
-                idx[j] = child_index(P.Gbase[gcell[j]], mask[j], key);
+                idx[j] = child_index(base, mask, key);
                 more = true;
             }
         }
-        // marked fine levels lc-1 .. 0 through the child links; the next
-        // level's first-child index is read beside the block
+        // marked fine levels lc-1 .. 0 through the child links
         for (int d = lc - 1; more && d >= 0; --d) {
-            uint32_t nbase[K];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (!act[j]) continue;
                 b0[j] = P.blocks[d][2 * (size_t)idx[j]];
                 b1[j] = P.blocks[d][2 * (size_t)idx[j] + 1];
-                nbase[j] = d > 0 ? P.cbase[d][idx[j]] : 0u;
             }
             const float s = pow2neg(d);
             more = false;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (!act[j]) continue;
+                if (d == 0) {
+                    f[j] += tri_level(b0[j], b1[j], ux[j], uy[j], uz[j], s);
+                    continue;
+                }
+                uint32_t base;
+                const uint32_t mask = link_decode(b0[j], b1[j], base);
                 f[j] += tri_level(b0[j], b1[j], ux[j], uy[j], uz[j], s);
-                if (d == 0) continue;
-                mask[j] = block_mask(b0[j], b1[j]);
-                const uint32_t key = child_key((int)ux[j], (int)uy[j], (int)uz[j], d - 1);
-                act[j] = (mask[j] >> key) & 1u;
+                const uint32_t key = child_key(cx[j], cy[j], cz[j], d - 1);
+                act[j] = (mask >> key) & 1u;
                 if (act[j]) {
-                    idx[j] = child_index(nbase[j], mask[j], key);
+                    idx[j] = child_index(base, mask, key);
                     more = true;
                 }
             }
@@ -319,25 +365,25 @@ __device__ inline float eval_query(const Loader& load, const EvalParams& P, int6
     if (!query_front(load, P, i, f, ux, uy, uz)) return f;
     const int lc = P.lc;
     const float sc = pow2neg(lc);
-    const uint32_t gcell = (((int)(uz * sc) * P.Rc + (int)(uy * sc)) * P.Rc + (int)(ux * sc));
-    float4 b0 = P.G[2 * (size_t)gcell], b1 = P.G[2 * (size_t)gcell + 1];
-    f = tri_level(b0, b1, ux, uy, uz, sc);
-    if (lc == 0) return f;
-    uint32_t mask = block_mask(b0, b1);
     const int cx = (int)ux, cy = (int)uy, cz = (int)uz;
+    const uint32_t g = g_cell(P, cx, cy, cz);
+    float4 b0 = P.G[2 * (size_t)g], b1 = P.G[2 * (size_t)g + 1];
+    if (lc == 0) return tri_level(b0, b1, ux, uy, uz, sc);
+    uint32_t base;
+    uint32_t mask = link_decode(b0, b1, base);
+    f = tri_level(b0, b1, ux, uy, uz, sc);
     uint32_t key = child_key(cx, cy, cz, lc - 1);
     if (!((mask >> key) & 1u)) return f;
-    uint32_t idx = child_index(P.Gbase[gcell], mask, key);
+    uint32_t idx = child_index(base, mask, key);
     for (int d = lc - 1; d >= 0; --d) {
         b0 = P.blocks[d][2 * (size_t)idx];
         b1 = P.blocks[d][2 * (size_t)idx + 1];
-        const uint32_t nbase = d > 0 ? P.cbase[d][idx] : 0u;
+        if (d == 0) return f + tri_level(b0, b1, ux, uy, uz, pow2neg(d));
+        mask = link_decode(b0, b1, base);
         f += tri_level(b0, b1, ux, uy, uz, pow2neg(d));
-        if (d == 0) break;
-        mask = block_mask(b0, b1);
         key = child_key(cx, cy, cz, d - 1);
         if (!((mask >> key) & 1u)) break;
-        idx = child_index(nbase, mask, key);
+        idx = child_index(base, mask, key);
     }
     return f;
 }
@@ -368,15 +414,15 @@ __global__ __launch_bounds__(kT) void k_eval_batch(const EvalParams* P, const Ba
 // exactly those 64, so the fine-level instructions run on full wavefronts.
 //
 // Entries are the query index within the launch (at most 2^31 queries per
-// launch), the grid coordinates u, the partial sum, the collapsed cell and
-// its child mask.  Points on the upper faces (u = R) take the clamped walk
+// launch), the grid coordinates u, the partial sum and the collapsed cell's
+// link word.  Points on the upper faces (u = R) take the clamped walk
 // with everything outside the grid.
 constexpr int kWaves = kT / 64;
 constexpr int kQueue = 128;   // at most 63 pending + 64 new entries
 
 struct WaveQueue {
     float ux[kWaves][kQueue], uy[kWaves][kQueue], uz[kWaves][kQueue], f[kWaves][kQueue];
-    uint32_t i[kWaves][kQueue], gcell[kWaves][kQueue], mask[kWaves][kQueue];
+    uint32_t i[kWaves][kQueue], base[kWaves][kQueue], mask[kWaves][kQueue];
 };
 
 __device__ inline void lds_fence() {
@@ -386,21 +432,22 @@ __device__ inline void lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// fine levels lc-1 .. 0 for one queued query
-__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint32_t gcell,
+// fine levels lc-1 .. 0 for one queued query (base, mask: the collapsed
+// cell's link word)
+__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint32_t base,
                                   uint32_t mask) {
     const int cx = (int)ux, cy = (int)uy, cz = (int)uz;   // u >= 0: truncation is floor
-    uint32_t idx = child_index(P.Gbase[gcell], mask, child_key(cx, cy, cz, P.lc - 1));
+    uint32_t idx = child_index(base, mask, child_key(cx, cy, cz, P.lc - 1));
     for (int d = P.lc - 1; d >= 0; --d) {
-        const float4 b0 = P.blocks[d][2 * (size_t)idx];
-        const float4 b1 = P.blocks[d][2 * (size_t)idx + 1];
-        const uint32_t nbase = d > 0 ? P.cbase[d][idx] : 0u;
+        float4 b0 = P.blocks[d][2 * (size_t)idx];
+        float4 b1 = P.blocks[d][2 * (size_t)idx + 1];
+        if (d == 0) return f + tri_level(b0, b1, ux, uy, uz, pow2neg(d));
+        uint32_t nbase;
+        const uint32_t m = link_decode(b0, b1, nbase);
         f += tri_level(b0, b1, ux, uy, uz, pow2neg(d));
-        if (d == 0) break;
-        mask = block_mask(b0, b1);
         const uint32_t key = child_key(cx, cy, cz, d - 1);
-        if (!((mask >> key) & 1u)) break;
-        idx = child_index(nbase, mask, key);
+        if (!((m >> key) & 1u)) break;
+        idx = child_index(nbase, m, key);
     }
     return f;
 }
@@ -418,13 +465,16 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
         const int64_t i = c + lane;
         bool push = false;
         float f = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
-        uint32_t gcell = 0, mask = 0;
+        uint2 lk = make_uint2(0u, 0u);
         if (i < N && query_front(load, P, off + i, f, ux, uy, uz)) {
-            gcell = (((int)(uz * sc) * P.Rc + (int)(uy * sc)) * P.Rc + (int)(ux * sc));
-            const float4 g0 = P.G[2 * (size_t)gcell], g1 = P.G[2 * (size_t)gcell + 1];
+            const int cx = (int)ux, cy = (int)uy, cz = (int)uz;
+            const uint32_t g = g_cell(P, cx, cy, cz);
+            float4 g0 = P.G[2 * (size_t)g], g1 = P.G[2 * (size_t)g + 1];
+            if (lc > 0) {
+                lk.y = link_decode(g0, g1, lk.x);
+                push = (lk.y >> child_key(cx, cy, cz, lc - 1)) & 1u;
+            }
             f = tri_level(g0, g1, ux, uy, uz, sc);
-            mask = block_mask(g0, g1);
-            if (lc > 0) push = (mask >> child_key((int)ux, (int)uy, (int)uz, lc - 1)) & 1u;
         }
         const uint64_t bal = __ballot(push);
         if (push) {
@@ -434,8 +484,8 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
             Q.uz[w][pos] = uz;
             Q.f[w][pos] = f;
             Q.i[w][pos] = (uint32_t)i;
-            Q.gcell[w][pos] = gcell;
-            Q.mask[w][pos] = mask;
+            Q.base[w][pos] = lk.x;
+            Q.mask[w][pos] = lk.y;
         } else if (i < N) {
             out[off + i] = f;
         }
@@ -443,7 +493,7 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
         if (qn >= 64) {
             lds_fence();
             const int pos = qn - 64 + lane;
-            const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.gcell[w][pos],
+            const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.base[w][pos],
                                       Q.mask[w][pos]);
             out[off + Q.i[w][pos]] = r;
             qn -= 64;
@@ -453,7 +503,7 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
     if (qn > 0) {
         lds_fence();
         if (lane < qn) {
-            const float r = fine_walk(P, Q.ux[w][lane], Q.uy[w][lane], Q.uz[w][lane], Q.f[w][lane], Q.gcell[w][lane],
+            const float r = fine_walk(P, Q.ux[w][lane], Q.uy[w][lane], Q.uz[w][lane], Q.f[w][lane], Q.base[w][lane],
                                       Q.mask[w][lane]);
             out[off + Q.i[w][lane]] = r;
         }
@@ -496,11 +546,11 @@ __global__ void k_cell_blocks(int Rc, const float* lat, float* blocks) {
     }
 }
 
-// every marked cell of the child level links into its parent: the smallest
-// child index and the bit of its key (marked cells only have marked parents,
-// so the parent entry exists)
+// every marked cell of the child level links into its parent's link word:
+// the smallest child index and the bit of its key (marked cells only have
+// marked parents, so the parent entry exists)
 __global__ void k_link(int64_t n, const uint32_t* codes, const RankWord* parent_cells, int parent_dense_R,
-                       uint32_t* parent_base, uint32_t* parent_mask, int* err) {
+                       uint2* parent_link, int* err) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t c = codes[i], p = c >> 3;
@@ -513,16 +563,26 @@ __global__ void k_link(int64_t n, const uint32_t* codes, const RankWord* parent_
         slot = bit_find(parent_cells, p);
         if (slot == ITS_NONE) { atomicOr(err, 1); return; }
     }
-    atomicMin(&parent_base[slot], (uint32_t)i);
-    atomicOr(&parent_mask[slot], 1u << (c & 7));
+    atomicMin(&parent_link[slot].x, (uint32_t)i);
+    atomicOr(&parent_link[slot].y, 1u << (c & 7));
 }
 
-// child mask bit k into the lowest mantissa bit of corner value k
-__global__ void k_embed_mask(int64_t n, const uint32_t* mask, uint32_t* blocks) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t m = mask[i];
-    for (int k = 0; k < 8; ++k) blocks[8 * i + k] = (blocks[8 * i + k] & ~1u) | ((m >> k) & 1u);
+// link words before linking: no child
+__global__ void k_link_clear(int64_t n, uint2* link) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        link[i] = make_uint2(ITS_NONE, 0u);
+}
+
+// link word of each cell into the 5 low bits of its 8 corner values (value
+// rounded to the nearest multiple of 32 ulps first, halves away from zero)
+__global__ void k_embed_link(int64_t n, const uint2* link, uint32_t* blocks) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint2 L = link[i];
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t u = (blocks[8 * i + k] + 16u) & ~31u;
+            blocks[8 * i + k] = u | ((L.y >> k) & 1u) | (((L.x >> (4 * k)) & 15u) << 1);
+        }
+    }
 }
 
 __global__ void k_emit_codes(const RankWord* g, int64_t nwords, uint32_t* out) {
@@ -549,12 +609,12 @@ EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_v
         P.blocks[d] = s->lv[d].blocks.p;
     }
     P.G = s->dense_blocks.p;
-    P.Gbase = s->dense_base.p;
-    for (int d = 0; d < s->depth; ++d) P.cbase[d] = s->lv[d].cbase.p;
     P.L = s->depth;
     P.R = s->R;
     P.lc = s->dense_level;
     P.Rc = s->lv[s->dense_level].R;
+    P.rc_shift = 0;
+    while ((1 << P.rc_shift) < P.Rc) ++P.rc_shift;
     P.ox = (float)s->origin[0];
     P.oy = (float)s->origin[1];
     P.oz = (float)s->origin[2];
@@ -653,22 +713,23 @@ void field_build(its_svo s) {
         hipLaunchKernelGGL(k_cell_blocks, dim3((unsigned)std::min<int64_t>(blocks_for(nc, kT), 65536)), dim3(kT), 0, st,
                            Rc, lat.p, (float*)s->dense_blocks.p);
         ITS_LAUNCH_CHECK();
-        s->dense_base.ensure(nc);
-        ITS_HIP(hipMemsetAsync(s->dense_base.p, 0xFF, nc * sizeof(uint32_t), st));   // ITS_NONE
+    }
+    if (lc == 0) {   // no linked levels: the values stay exact
+        ITS_HIP(hipStreamSynchronize(st));
+        s->field_ready = true;
+        return;
     }
     // child links of every level above the finest, down to the collapsed one:
-    // first-child index per cell, and the child mask folded into its block
+    // a link word (first marked child, child mask) per cell, then folded into
+    // the cell's block
     const int64_t nc_dense = (int64_t)s->lv[lc].R * s->lv[lc].R * s->lv[lc].R;
-    DBuf<uint32_t> dense_mask(nc_dense);
-    ITS_HIP(hipMemsetAsync(dense_mask.p, 0, nc_dense * sizeof(uint32_t), st));
-    std::vector<DBuf<uint32_t>> masks(lc);
-    for (int d = 1; d < lc; ++d) {
-        Level& L = s->lv[d];
-        const int64_t m = std::max<int64_t>(L.n_cells, 1);
-        L.cbase.ensure(m);
-        ITS_HIP(hipMemsetAsync(L.cbase.p, 0xFF, m * sizeof(uint32_t), st));
-        masks[d].ensure(m);
-        ITS_HIP(hipMemsetAsync(masks[d].p, 0, m * sizeof(uint32_t), st));
+    std::vector<DBuf<uint2>> links(lc + 1);
+    for (int d = 1; d <= lc; ++d) {
+        const int64_t m = d == lc ? nc_dense : std::max<int64_t>(s->lv[d].n_cells, 1);
+        links[d].alloc(m);
+        hipLaunchKernelGGL(k_link_clear, dim3((unsigned)std::min<int64_t>(blocks_for(m, kT), 65536)), dim3(kT), 0, st,
+                           m, links[d].p);
+        ITS_LAUNCH_CHECK();
     }
     DBuf<int> err(1);
     ITS_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
@@ -680,23 +741,17 @@ void field_build(its_svo s) {
         ITS_LAUNCH_CHECK();
         const bool dense_parent = (d + 1 == lc);
         hipLaunchKernelGGL(k_link, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, codes.p,
-                           s->lv[d + 1].cells.p, dense_parent ? s->lv[lc].R : 0,
-                           dense_parent ? s->dense_base.p : s->lv[d + 1].cbase.p,
-                           dense_parent ? dense_mask.p : masks[d + 1].p, err.p);
+                           s->lv[d + 1].cells.p, dense_parent ? s->lv[lc].R : 0, links[d + 1].p, err.p);
         ITS_LAUNCH_CHECK();
         ITS_HIP(hipStreamSynchronize(st));
     }
     ITS_REQUIRE(read1(err.p, st) == 0, ITS_E_STATE, "marked cell without a marked parent");
-    if (lc > 0) {
-        hipLaunchKernelGGL(k_embed_mask, dim3(blocks_for(nc_dense, kT)), dim3(kT), 0, st, nc_dense, dense_mask.p,
-                           (uint32_t*)s->dense_blocks.p);
-        ITS_LAUNCH_CHECK();
-    }
-    for (int d = 1; d < lc; ++d) {
-        Level& L = s->lv[d];
-        if (L.n_cells == 0) continue;
-        hipLaunchKernelGGL(k_embed_mask, dim3(blocks_for(L.n_cells, kT)), dim3(kT), 0, st, L.n_cells, masks[d].p,
-                           (uint32_t*)L.blocks.p);
+    for (int d = 1; d <= lc; ++d) {
+        const int64_t m = d == lc ? nc_dense : s->lv[d].n_cells;
+        if (m == 0) continue;
+        uint32_t* blocks = (uint32_t*)(d == lc ? s->dense_blocks.p : s->lv[d].blocks.p);
+        hipLaunchKernelGGL(k_embed_link, dim3((unsigned)std::min<int64_t>(blocks_for(m, kT), 65536)), dim3(kT), 0, st,
+                           m, (const uint2*)links[d].p, blocks);
         ITS_LAUNCH_CHECK();
     }
     ITS_HIP(hipStreamSynchronize(st));

@@ -22,9 +22,7 @@ struct Level {
     // evaluation field: cells whose corners include a vertex
     DBuf<RankWord> cells;   // R^3 bits by Morton code
     DBuf<float4> blocks;    // 2 float4 (8 corner coefficients) per marked cell; on the
-                            // levels linked below the collapsed one, the cell's child mask
-                            // rides in the lowest mantissa bit of each coefficient
-    DBuf<uint32_t> cbase;   // index of each marked cell's first marked child (linked levels)
+                            // linked levels the 5 low bits of each carry the child link
     int64_t n_cells = 0;
 };
 
@@ -71,8 +69,8 @@ struct its_svo_s {
     its::DBuf<double> lambda64;
     bool has_lambda = false;
     bool field_ready = false;
-    its::DBuf<float4> dense_blocks;   // per dense_level cell: corner values of levels >= dense_level
-    its::DBuf<uint32_t> dense_base;   // per dense_level cell: first marked child
+    its::DBuf<float4> dense_blocks;   // per dense_level cell: corner values of levels >= dense_level,
+                                      // child link in the 5 low bits of each
     int dense_level = 0;
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};
