@@ -12,8 +12,11 @@
 // Every scalar of the recurrence is computed by all threads from the same
 // block sums, so all threads take the same branches and reach every barrier.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <exception>
+#include <thread>
 #include <vector>
 
 #include "fit.hpp"
@@ -257,11 +260,15 @@ struct its_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t side = nullptr;       // the one-launch fit runs here, beside the large meshes
+    static constexpr int kWorkers = 3;   // large meshes: one host thread and stream each
+    hipStream_t work[kWorkers] = {};
     std::vector<its::FitDesc> host;   // the descriptors the graph was captured with
     ~its_batch_s() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (side) (void)hipStreamDestroy(side);
+        for (hipStream_t w : work)
+            if (w) (void)hipStreamDestroy(w);
     }
 };
 
@@ -358,11 +365,46 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
         ITS_HIP(hipStreamSynchronize(st));   // inputs written on the context stream are complete
         ITS_HIP(hipGraphLaunch(b->gexec, b->side));
     }
+    // Large meshes: the multi-workgroup solver, whose inner loop is host
+    // driven, from up to kWorkers host threads with a stream each, largest
+    // first.  Each solve only touches its own octree's state, so the results
+    // do not depend on the interleaving.
     std::vector<its_lscg_stats> big_stats(big.size());
-    for (size_t q = 0; q < big.size(); ++q) {
-        its_svo s = b->svos[big[q]];
-        s->lambda64.ensure(s->num_verts);
-        lscg_solve(s, s->rhs.p, s->lambda64.p, tol, max_iter, warm, &big_stats[q]);
+    if (!big.empty()) {
+        std::vector<size_t> order(big.size());
+        for (size_t q = 0; q < big.size(); ++q) {
+            order[q] = q;
+            its_svo s = b->svos[big[q]];
+            (void)fit_get(s);   // solver state built here, on the context stream
+            s->lambda64.ensure(s->num_verts);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+            return b->svos[big[x]]->num_verts > b->svos[big[y]]->num_verts;
+        });
+        ITS_HIP(hipStreamSynchronize(st));
+        const int nw = (int)std::min<size_t>(its_batch_s::kWorkers, big.size());
+        for (int w = 0; w < nw; ++w)
+            if (!b->work[w]) ITS_HIP(hipStreamCreateWithFlags(&b->work[w], hipStreamNonBlocking));
+        std::atomic<size_t> next{0};
+        std::vector<std::exception_ptr> errs(nw);
+        const int device = b->ctx->device;
+        auto worker = [&](int w) {
+            try {
+                ITS_HIP(hipSetDevice(device));
+                for (size_t k; (k = next.fetch_add(1)) < order.size();) {
+                    const size_t q = order[k];
+                    its_svo s = b->svos[big[q]];
+                    lscg_solve(s, s->rhs.p, s->lambda64.p, tol, max_iter, warm, &big_stats[q], b->work[w]);
+                }
+            } catch (...) {
+                errs[w] = std::current_exception();
+            }
+        };
+        std::vector<std::thread> threads;
+        for (int w = 0; w < nw; ++w) threads.emplace_back(worker, w);
+        for (std::thread& t : threads) t.join();
+        for (const std::exception_ptr& e : errs)
+            if (e) std::rethrow_exception(e);
     }
     if (ns > 0) ITS_HIP(hipStreamSynchronize(b->side));
     ITS_HIP(hipStreamSynchronize(st));

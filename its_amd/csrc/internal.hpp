@@ -91,8 +91,10 @@ void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box
                        float* f, hipStream_t st);
 // lscg.hip
 void fit_release(Fit* f);
+// on `stream` when given (the octree's solver state must exist already,
+// fit_get), else on the context stream
 void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int64_t max_iter, bool warm,
-                its_lscg_stats* st);
+                its_lscg_stats* st, hipStream_t stream = nullptr);
 void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose);
 // mc.hip
 void mc_extract(its_svo s, const double origin[3], const double width[3], const int res[3], double iso,

@@ -759,10 +759,10 @@ void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose) 
 }
 
 void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int64_t max_iter, bool warm,
-                its_lscg_stats* out) {
+                its_lscg_stats* out, hipStream_t stream) {
     auto t0 = std::chrono::steady_clock::now();
     Fit* f = fit_get(s);
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream ? stream : s->ctx->stream;
     const int64_t n = f->n;
     if (max_iter <= 0) max_iter = 2 * n;
     if (!warm) ITS_HIP(hipMemsetAsync(d_lambda, 0, n * sizeof(double), st));

@@ -28,6 +28,8 @@
 #include <array>
 #include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <cmath>
 #include <fstream>
 #include <limits>
 #include <sstream>
@@ -37,6 +39,7 @@
 #include <vector>
 
 #include "its_c.h"
+#include "its_io.hpp"
 
 namespace its_host {
 
@@ -184,6 +187,28 @@ public:
         return out;
     }
 
+    // cubes of every level, one OBJ per level: <dir>/svo_1.obj (finest) ..
+    // svo_<treeDepth>.obj (SVO.cu:1266-1318, saveSVO)
+    void saveSVO(const std::string& dir) const {
+        const size_t n = numTreeNodes;
+        std::vector<uint32_t> mc(n), par(n), ch(8 * n), nb(27 * n);
+        std::vector<uint8_t> leaf(n);
+        std::vector<double> org(3 * n), wd(n);
+        check(its_svo_export_nodes(svo_, mc.data(), leaf.data(), org.data(), wd.data(), par.data(), ch.data(),
+                                   nb.data()));
+        size_t first = 0;
+        for (int d = 0; d < treeDepth; ++d) {
+            std::vector<io::P3> o;
+            std::vector<double> w;
+            for (size_t i = first; i < first + depthNumNodes[d]; ++i) {
+                o.push_back({org[3 * i], org[3 * i + 1], org[3 * i + 2]});
+                w.push_back(wd[i]);
+            }
+            io::save_cubes(dir + "/svo_" + std::to_string(d + 1) + ".obj", o, w);
+            first += depthNumNodes[d];
+        }
+    }
+
     its_svo handle() const { return svo_; }
     its_ctx ctx_() const { return default_ctx(); }
     double voxelWidth() const { return voxel_width_; }
@@ -201,6 +226,8 @@ class ThinShells {
 public:
     int treeDepth = 0;
     std::string uniformDir = "non-uniform";
+    std::string noiseDir = "non-noise";
+    std::string modelName = "model";
 
     // BaseModel(vector<V3d> verts, vector<V3i> faces): no normalisation
     ThinShells(const std::vector<V3d>& verts, const std::vector<V3i>& faces, int gx, int gy, int gz,
@@ -215,6 +242,9 @@ public:
         std::vector<V3d> V;
         std::vector<V3i> F;
         readOFF(filename, V, F);
+        const size_t slash = filename.find_last_of("/\\");
+        modelName = filename.substr(slash == std::string::npos ? 0 : slash + 1);
+        modelName = modelName.substr(0, modelName.find_last_of('.'));
         init(V, F, is2UnitCube, scaleFactor);
     }
 
@@ -226,18 +256,86 @@ public:
                                   (int64_t)(mF.size() / 3), sdfVal.data()));
         lambda.assign(n, 0.0);
         check(its_lscg_solve(c, svo.handle(), nullptr, lambda.data(), tol, 0, 0, &lastStats));
-        const int64_t nv = (int64_t)(mV.size() / 3);
-        std::vector<float> f(nv);
-        const double end[3] = {bbox.boxEnd[0], bbox.boxEnd[1], bbox.boxEnd[2]};
-        check(its_bspline_eval_f64pts(c, svo.handle(), mV.data(), nv, end, ITS_EVAL_SUPPORT, 0.f, f.data()));
-        bSplineVal.assign(f.begin(), f.end());
+        const int64_t nv = (int64_t)(mV.size() / 3), nf = (int64_t)(mF.size() / 3);
         double iso[2];
         int64_t npts = 0;
-        check(its_shell_iso(c, svo.handle(), mV.data(), nv, mF.data(), (int64_t)(mF.size() / 3), iso, &npts));
+        check(its_shell_iso(c, svo.handle(), mV.data(), nv, mF.data(), nf, iso, &npts));
         innerShellIsoVal = iso[0];
         outerShellIsoVal = iso[1];
         numInterPoints = npts;
+        // the latent points (cpIntersectionPoints, ThinShells.cpp:38-479):
+        // edge points sorted by the reference's fuzzy lexicographic order with
+        // exact duplicates removed, then the face points
+        int64_t ne = 0, nfc = 0;
+        check(its_shell_points(c, svo.handle(), mV.data(), nv, mF.data(), nf, nullptr, 0, &ne, &nfc));
+        std::vector<double> P(3 * (size_t)(ne + nfc));
+        if (ne + nfc)
+            check(its_shell_points(c, svo.handle(), mV.data(), nv, mF.data(), nf, P.data(), ne + nfc, &ne, &nfc));
+        std::vector<io::P3> edge(ne);
+        for (int64_t i = 0; i < ne; ++i) edge[i] = {P[3 * i], P[3 * i + 1], P[3 * i + 2]};
+        std::sort(edge.begin(), edge.end(), [](const io::P3& a, const io::P3& b) {
+            for (int k = 0; k < 3; ++k) {
+                if (std::fabs(a[k] - b[k]) < 1e-9) continue;
+                return a[k] < b[k];
+            }
+            return false;
+        });
+        edge.erase(std::unique(edge.begin(), edge.end()), edge.end());
+        allInterPoints.assign(edge.begin(), edge.end());
+        for (int64_t i = ne; i < ne + nfc; ++i) allInterPoints.push_back({P[3 * i], P[3 * i + 1], P[3 * i + 2]});
+        // f at the model vertices and the latent points (cpLatentBSplineValue)
+        std::vector<double> pts(mV);
+        for (const io::P3& q : allInterPoints) pts.insert(pts.end(), q.begin(), q.end());
+        const int64_t np = (int64_t)(pts.size() / 3);
+        std::vector<float> f(np);
+        const double end[3] = {bbox.boxEnd[0], bbox.boxEnd[1], bbox.boxEnd[2]};
+        check(its_bspline_eval_f64pts(c, svo.handle(), pts.data(), np, end, ITS_EVAL_SUPPORT, 0.f, f.data()));
+        bSplineVal.assign(f.begin(), f.end());
     }
+
+    // ---- text outputs (ThinShells.cpp:1017-1081); an empty name gives
+    // output/<model>/<uniformDir>/<noiseDir>/<treeDepth>/<file>
+    void saveSDFValue(const std::string& filename = "") const { io::save_values(outPath(filename, "SDFValue.txt"), sdfVal); }
+    void saveCoefficients(const std::string& filename = "") const {
+        io::save_values(outPath(filename, "Coefficients.txt"), lambda);
+    }
+    void saveLatentPoint(const std::string& filename = "") const {
+        std::vector<io::P3> pts;
+        for (size_t i = 0; i < mV.size() / 3; ++i) pts.push_back({mV[3 * i], mV[3 * i + 1], mV[3 * i + 2]});
+        pts.insert(pts.end(), allInterPoints.begin(), allInterPoints.end());
+        std::ofstream out = io::open_out(outPath(filename, "latent_point.xyz"));
+        io::write_point_cloud(out, pts, ".xyz");
+    }
+    void saveBSplineValue(const std::string& filename = "") const {
+        io::save_values_fixed9(outPath(filename, "BSplineValue.txt"), bSplineVal);
+    }
+
+    // Halton points in the model box scaled by scaleFactor about its centre
+    // and moved by dis, also written to filename (".obj" or ".xyz")
+    // (BaseModel::generateUniformRandomPoints, BaseModel.cpp:246-264)
+    std::vector<V3d> generateUniformRandomPoints(const std::string& filename, size_t numPoints, double scaleFactor,
+                                                 const V3d& dis) const {
+        const std::vector<io::P3> p =
+            io::uniform_points({bbox.boxOrigin[0], bbox.boxOrigin[1], bbox.boxOrigin[2]},
+                               {bbox.boxEnd[0], bbox.boxEnd[1], bbox.boxEnd[2]}, numPoints, scaleFactor,
+                               {dis[0], dis[1], dis[2]});
+        if (!filename.empty()) io::save_point_cloud(filename, p);
+        std::vector<V3d> out(p.size());
+        for (size_t i = 0; i < p.size(); ++i) out[i] = V3d(p[i][0], p[i][1], p[i][2]);
+        return out;
+    }
+
+    std::vector<V3d> getModelVerts() const {
+        std::vector<V3d> v(mV.size() / 3);
+        for (size_t i = 0; i < v.size(); ++i) v[i] = V3d(mV[3 * i], mV[3 * i + 1], mV[3 * i + 2]);
+        return v;
+    }
+    std::vector<V3i> getModelFaces() const {
+        std::vector<V3i> f(mF.size() / 3);
+        for (size_t i = 0; i < f.size(); ++i) f[i] = V3i(mF[3 * i], mF[3 * i + 1], mF[3 * i + 2]);
+        return f;
+    }
+    const std::vector<io::P3>& latentPoints() const { return allInterPoints; }
 
     std::array<double, 2> getShellIsoVal() const { return {innerShellIsoVal, outerShellIsoVal}; }
 
@@ -289,7 +387,13 @@ private:
     std::vector<int32_t> mF;
     AABox<V3d> bbox;
     std::vector<double> sdfVal, lambda, bSplineVal;
+    std::vector<io::P3> allInterPoints;
     double innerShellIsoVal = -kInf, outerShellIsoVal = -kInf;
+
+    std::string outPath(const std::string& filename, const std::string& name) const {
+        if (!filename.empty()) return filename;
+        return "output/" + modelName + "/" + uniformDir + "/" + noiseDir + "/" + std::to_string(treeDepth) + "/" + name;
+    }
     its_lscg_stats lastStats{};
 
     void init(const std::vector<V3d>& verts, const std::vector<V3i>& faces, bool unit, double scale) {
@@ -377,15 +481,89 @@ inline int64_t marching_cubes(const SparseVoxelOctree& svo, const V3d& gridOrigi
     if (st.num_verts)
         check(its_mc_extract(svo.ctx_(), svo.handle(), o, w, res, isoVal, tri.data(), st.num_verts, &st));
     if (!filename.empty()) {
-        std::ofstream out(filename);
-        if (!out) throw Error(ITS_E_INVALID, "cannot write " + filename);
-        for (int64_t i = 0; i < st.num_verts; ++i)
-            out << "v " << tri[3 * i] << ' ' << tri[3 * i + 1] << ' ' << tri[3 * i + 2] << '\n';
-        for (int64_t i = 0; i + 2 < st.num_verts; i += 3) out << "f " << i + 1 << ' ' << i + 2 << ' ' << i + 3 << '\n';
+        std::vector<io::P3> soup(st.num_verts);
+        for (int64_t i = 0; i < st.num_verts; ++i) soup[i] = {tri[3 * i], tri[3 * i + 1], tri[3 * i + 2]};
+        io::save_soup_obj(filename, soup);
     }
     return st.num_verts;
 }
 }  // namespace MC
+
+// Point in/out validation (reference main.cpp:44-134, testPointInOut): the
+// shell classification of Halton points against the sign of the exact
+// signed distance (its_signed_distance: binary64 nearest triangle, winding
+// number sign), which stands in for the reference's fcpw scene.  Prints the
+// reference's report lines and returns the figures.  queryResFile, unused by
+// the reference, receives the points whose classes differ (".xyz" / ".obj")
+// when it is not empty.
+struct PointInOutReport {
+    size_t numPoints = 0, correct = 0, numErrorPoints = 0;
+    double correctRate = 0;         // percent
+    double oursSeconds = 0;         // one multiPointQuery over all points
+    double exactSeconds = 0;        // exact signed distance over all points
+    double oursPlusExactSeconds = 0;  // ours + exact distance of the differing points
+};
+
+inline PointInOutReport testPointInOut(ThinShells& thinShell, size_t numPoints, const std::string& queryFile,
+                                       const std::string& queryResFile, int query_distance) {
+    PointInOutReport r;
+    r.numPoints = numPoints;
+    std::printf("\n[Test] Point INSIDE or OUTSIDE surface\n");
+    const std::vector<V3d> pts = thinShell.generateUniformRandomPoints(queryFile, numPoints, query_distance, V3d(0, 0, 0));
+    const int session = 3;
+    double time_1 = 0;
+    const std::vector<int> ours = thinShell.multiPointQuery(pts, time_1, session, Test::CPU);
+    if (ours.empty()) return r;
+    r.oursSeconds = time_1;
+    std::printf("-- [Ours]: Multi points query spent %lf us.\n", time_1 * 1e6 / (double)numPoints);
+
+    std::vector<double> V, P(3 * numPoints);
+    std::vector<int32_t> F;
+    for (const V3d& v : thinShell.getModelVerts()) V.insert(V.end(), {v[0], v[1], v[2]});
+    for (const V3i& f : thinShell.getModelFaces()) F.insert(F.end(), {f[0], f[1], f[2]});
+    for (size_t i = 0; i < numPoints; ++i)
+        for (int a = 0; a < 3; ++a) P[3 * i + a] = pts[i][a];
+    std::vector<double> sdf(numPoints);
+    auto exact = [&](const double* p, int64_t n, double* out) {
+        check(its_signed_distance(thinShell.octree().ctx_(), V.data(), (int64_t)(V.size() / 3), F.data(),
+                                  (int64_t)(F.size() / 3), p, n, out));
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    exact(P.data(), (int64_t)numPoints, sdf.data());
+    r.exactSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("-- [Exact SDF]: Multi points query spent %lf us.\n", r.exactSeconds * 1e6 / (double)numPoints);
+
+    std::vector<double> errorPts;
+    std::vector<io::P3> errorOut;
+    for (size_t i = 0; i < numPoints; ++i) {
+        const int e = sdf[i] > 0 ? 1 : (sdf[i] < 0 ? -1 : 0);
+        if (ours[i] == e) {
+            ++r.correct;
+        } else {
+            errorPts.insert(errorPts.end(), {pts[i][0], pts[i][1], pts[i][2]});
+            errorOut.push_back({pts[i][0], pts[i][1], pts[i][2]});
+        }
+    }
+    r.numErrorPoints = errorOut.size();
+    r.correctRate = numPoints ? (double)r.correct * 100.0 / (double)numPoints : 0.0;
+    std::printf("-- Correct rate = %lf%%\n", r.correctRate);
+    if (!queryResFile.empty()) io::save_point_cloud(queryResFile, errorOut);
+
+    // the differing points through the exact distance, averaged over sessions
+    double time_3 = 0;
+    if (r.numErrorPoints) {
+        std::vector<double> d(r.numErrorPoints);
+        for (int k = 0; k < session; ++k) {
+            auto t1 = std::chrono::steady_clock::now();
+            exact(errorPts.data(), (int64_t)r.numErrorPoints, d.data());
+            time_3 += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        }
+        time_3 /= session;
+    }
+    r.oursPlusExactSeconds = time_1 + time_3;
+    std::printf("-- [Ours+Exact]: Multi points query spent %lf us.\n", r.oursPlusExactSeconds * 1e6 / (double)numPoints);
+    return r;
+}
 
 inline void ThinShells::mcVisualization(const std::string& innerFilename, const V3i& innerResolution,
                                         const std::string& outerFilename, const V3i& outerResolution,

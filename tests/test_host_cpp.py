@@ -77,3 +77,72 @@ def test_demo_matches_python_path(ctx, tmp_path):
                                                               int((cls == 1).sum())]
     assert got["mc_verts"] == mst["num_verts"]
     svo.close()
+
+
+def _read(p):
+    with open(p) as fh:
+        return fh.read()
+
+
+@pytest.mark.gpu
+def test_demo_dumps_and_validation(ctx, tmp_path):
+    """The demo's text outputs (SDFValue, Coefficients, latent points, f at the
+    latent points, octree cubes per level, MC soup) and its point in/out
+    validation (main.cpp:44-134) against the same steps through the Python
+    binding, with the formats of oracle/formats.py."""
+    import formats as FMT
+    from its_amd import _capi, meshes
+    d = np.load(GOLDEN)
+    V, F = d["V"], d["F"].astype(np.int32)
+    off = tmp_path / "bunny.off"
+    _write_off(off, V, F)
+    R, nq = 32, 4000
+    dump, obj = tmp_path / "dump", tmp_path / "iso.obj"
+    print(f"[host demo dumps] bunny R={R}", flush=True)
+    r = subprocess.run([DEMO, str(off), str(R), str(nq), str(obj), str(dump)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+
+    Vn = _capi.model_to_unit_cube(V)
+    origin, end, width = _capi.uniform_bbox(Vn)
+    svo = _capi.Octree(ctx, meshes.triangles(Vn, F), origin, width, R)
+    b = svo.sdf_at_vertices(Vn, F)
+    lam, st = svo.lscg(b, tol=1e-6)
+    assert _read(dump / "SDFValue.txt") == FMT.values_text(b)
+    assert _read(dump / "Coefficients.txt") == FMT.values_text(lam)
+
+    # latent points: the model vertices, then the intersection points (edge
+    # points without exact duplicates, face points)
+    P, ne = svo.shell_points(Vn, F)
+    edge = np.unique(P[:ne], axis=0)
+    lines = _read(dump / "latent_point.xyz").splitlines()
+    assert len(lines) == len(Vn) + got["latent_points"]
+    assert len(edge) + len(P) - ne <= got["latent_points"] <= len(P)
+    assert lines[: len(Vn)] == FMT.point_cloud_text(Vn, ".xyz").splitlines()
+    assert set(lines[len(Vn):]) == set(FMT.point_cloud_text(np.concatenate([edge, P[ne:]]), ".xyz").splitlines())
+    bs = _read(dump / "BSplineValue.txt").splitlines()
+    assert len(bs) == len(lines) and len({len(l) for l in bs}) == 1   # aligned column
+    fV = svo.eval(Vn, end, _capi.EVAL_SUPPORT, 0.0).astype(np.float64)
+    assert [l.strip() for l in bs[: len(Vn)]] == ["%.9f" % v for v in fV]
+
+    nodes = svo.nodes()
+    first = 0
+    for lvl, cnt in enumerate(svo.depth_num_nodes()):
+        o, w = nodes["origin"][first:first + cnt], nodes["width"][first:first + cnt]
+        assert _read(dump / "svo" / f"svo_{lvl + 1}.obj") == FMT.cube_obj_text(o.tolist(), w.tolist())
+        first += cnt
+    tri, mst = svo.marching_cubes(origin, width, (R, R, R), 0.0)
+    assert _read(obj) == FMT.soup_obj_text(tri.astype(np.float64).tolist())
+
+    Q = np.array(FMT.uniform_points(list(origin), list(end), nq, 1.0, [0.0, 0.0, 0.0]))
+    assert _read(dump / "query.xyz") == FMT.point_cloud_text(Q, ".xyz")
+    inner, outer, _ = svo.shell_iso(Vn, F)
+    cls = _capi.classify(svo.eval(Q, end, _capi.EVAL_SUPPORT, np.float32(outer)), inner, outer)
+    exact = np.sign(ctx.signed_distance(Vn, F, Q)).astype(int)
+    correct = int((cls == exact).sum())
+    assert got["check_correct"] == correct and got["check_errors"] == nq - correct
+    assert got["check_rate"] == pytest.approx(100.0 * correct / nq, abs=1e-6)
+    assert len(_read(dump / "query_err.xyz").splitlines()) == nq - correct
+    assert correct > nq // 2
+    svo.close()
