@@ -89,6 +89,16 @@ int its_svo_export_nodes(its_svo svo, uint32_t* morton, uint8_t* is_leaf, double
 int its_svo_export_verts(its_svo svo, double* xyz, uint32_t* owner);
 /* sorted occupied-voxel Morton codes (the compacted level-0 array) */
 int its_svo_export_voxels(its_svo svo, uint32_t* morton);
+/* per vertex (global order): its lattice coordinates at its own level, the
+ * key the vertex order of depthNodeVertexArray sorts by (x, then y, then z;
+ * SVO.cu:1064-1096), and that level.  Either array may be NULL. */
+int its_svo_export_vert_lattice(its_svo svo, uint16_t* ijk, uint8_t* level);
+/* level-0 node edges, the reference's fineNodeEdgeArray (determineNodeEdge,
+ * sortEdge, uniqueEdge: src/cuAcc/SVO.cu:965-1008, 1033-1060, 1098-1110):
+ * end points (6 doubles per edge, lower end first) and the smallest index of
+ * the level-0 nodes containing the edge, in the reference's order.  ends and
+ * owner may be NULL to count; capacity is in edges. */
+int its_svo_export_fine_edges(its_svo svo, double* ends, uint32_t* owner, int64_t capacity, int64_t* count);
 
 /* ---- right-hand side: signed distance at every node vertex
  *      (ThinShells::cpSDFOfTreeNodes, ThinShells.cpp:481-509) ----

@@ -97,6 +97,8 @@ SIGNATURES = {
     "its_svo_export_nodes": [vp, vp, vp, vp, vp, vp, vp, vp],
     "its_svo_export_verts": [vp, vp, vp],
     "its_svo_export_voxels": [vp, vp],
+    "its_svo_export_fine_edges": [vp, vp, vp, C.c_int64, vp],
+    "its_svo_export_vert_lattice": [vp, vp, vp],
     "its_sdf_at_vertices": [vp, vp, _f64p, C.c_int64, _i32p, C.c_int64, vp],
     "its_signed_distance": [vp, _f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, _f64p],
     "its_lscg_solve": [vp, vp, vp, _f64p, C.c_double, C.c_int64, C.c_int, C.POINTER(LscgStats)],
@@ -338,6 +340,24 @@ class Octree:
         out = np.zeros(max(self.num_voxels, 1), np.uint32)
         check(lib().its_svo_export_voxels(self._h, out.ctypes.data_as(vp)))
         return out[: self.num_voxels]
+
+    def vert_lattice(self):
+        """Per vertex: lattice coordinates at its level (n, 3) and the level."""
+        n = max(self.num_verts, 1)
+        ijk = np.zeros((n, 3), np.uint16)
+        lev = np.zeros(n, np.uint8)
+        check(lib().its_svo_export_vert_lattice(self._h, ijk.ctypes.data_as(vp), lev.ctypes.data_as(vp)))
+        return ijk[: self.num_verts], lev[: self.num_verts]
+
+    def fine_edges(self):
+        """Level-0 node edges: (n, 2, 3) end points and owner node per edge."""
+        n = C.c_int64(0)
+        check(lib().its_svo_export_fine_edges(self._h, None, None, 0, C.byref(n)))
+        ends = np.zeros((max(n.value, 1), 2, 3))
+        owner = np.zeros(max(n.value, 1), np.uint32)
+        check(lib().its_svo_export_fine_edges(self._h, ends.ctypes.data_as(vp), owner.ctypes.data_as(vp), n.value,
+                                              C.byref(n)))
+        return ends[: n.value], owner[: n.value]
 
     def sdf_at_vertices(self, V, F):
         b = np.zeros(max(self.num_verts, 1))

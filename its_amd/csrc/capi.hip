@@ -273,6 +273,50 @@ int its_svo_export_nodes(its_svo s, uint32_t* morton, uint8_t* is_leaf, double* 
     ITS_API_END
 }
 
+int its_svo_export_vert_lattice(its_svo s, uint16_t* ijk, uint8_t* level) {
+    ITS_API_BEGIN
+    check_svo(s);
+    DeviceGuard g(s->ctx->device);
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = s->num_verts;
+    if (n == 0) return ITS_OK;
+    if (ijk) {
+        std::vector<uint16_t> x, y, z;
+        to_host(x, s->vx.p, n, st);
+        to_host(y, s->vy.p, n, st);
+        to_host(z, s->vz.p, n, st);
+        ITS_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < n; ++i) {
+            ijk[3 * i] = x[i];
+            ijk[3 * i + 1] = y[i];
+            ijk[3 * i + 2] = z[i];
+        }
+    }
+    if (level) {
+        ITS_HIP(hipMemcpyAsync(level, s->vlevel.p, n, hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    ITS_API_END
+}
+
+int its_svo_export_fine_edges(its_svo s, double* ends, uint32_t* owner, int64_t capacity, int64_t* count) {
+    ITS_API_BEGIN
+    check_svo(s);
+    DeviceGuard g(s->ctx->device);
+    hipStream_t st = s->ctx->stream;
+    DBuf<double> E;
+    DBuf<uint32_t> O;
+    const int64_t n = fine_edges(s, E, O, st);
+    if (count) *count = n;
+    if ((ends || owner) && n > 0) {
+        ITS_REQUIRE(n <= capacity, ITS_E_RANGE, "edge buffer too small");
+        if (ends) ITS_HIP(hipMemcpyAsync(ends, E.p, 6 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (owner) ITS_HIP(hipMemcpyAsync(owner, O.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+    }
+    ITS_API_END
+}
+
 int its_svo_export_verts(its_svo s, double* xyz, uint32_t* owner) {
     ITS_API_BEGIN
     check_svo(s);

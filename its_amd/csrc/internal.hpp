@@ -108,6 +108,9 @@ void nearest_normals(hipStream_t st, const double* d_V, const int32_t* d_F, cons
 void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
                 const float* z, float out_val, float* f, hipStream_t st);
 // shell.hip
+// level-0 node edges with owners in the reference's fineNodeEdgeArray order;
+// returns the count
+int64_t fine_edges(its_svo s, DBuf<double>& ends, DBuf<uint32_t>& owner, hipStream_t st);
 void shell_points(its_svo s, const double* d_V, const uint64_t* d_edges, int64_t ne, const double* d_tris, int64_t nT,
                   DBuf<double>& pts, int64_t* n_edge, int64_t* n_face, hipStream_t st);
 std::vector<uint64_t> mesh_edges(const int32_t* F, int64_t nF);

@@ -253,6 +253,125 @@ __global__ void k_face_points(int64_t nT, const double* tris, Fine S, const uint
 
 // Intersection points on the device; returns the count of edge points and
 // face points (edge points first).  d_V: nV x 3, d_tris: nT x 9 (binary64).
+// ---- fine node edges (determineNodeEdge, sortEdge, uniqueEdge:
+// reference SVO.cu:965-1008, 1033-1060, 1098-1110).  The reference sorts the
+// 12 edges of every level-0 node by their end points in the fuzzy 1e-9
+// lexicographic order, then by the smallest index of the nodes sharing the
+// edge, and keeps one per group of equal edges.  End points are lattice
+// points, so that order is the lattice order of the lower end (x, then y,
+// then z) and, at a shared lower end, z-edge < y-edge < x-edge.  Here every
+// edge is one bit of a lattice bitset in exactly that order,
+//   bit = ((X (R+1) + Y) (R+1) + Z) * 3 + slot,   slot 0 z, 1 y, 2 x,
+// its owner is the smallest level-0 node containing it (atomic minimum), and
+// its end points are the owner's origin plus 0 or the width per axis.
+namespace {
+__device__ inline void fine_xyz(uint32_t m, int& x, int& y, int& z) {
+    uint32_t ux, uy, uz;
+    demorton3(m, ux, uy, uz);
+    x = (int)ux;
+    y = (int)uy;
+    z = (int)uz;
+}
+__device__ inline uint64_t fedge_bit(int R, int X, int Y, int Z, int slot) {
+    const uint64_t D = (uint64_t)R + 1;
+    return (((uint64_t)X * D + (uint64_t)Y) * D + (uint64_t)Z) * 3u + (uint64_t)slot;
+}
+// the 12 edges of a cell: axis a (0 x, 1 y, 2 z) from the corner with
+// offsets (dx, dy, dz), 0 along a
+__device__ inline void cell_edge(int e, int& a, int off[3]) {
+    a = e >> 2;
+    const int u = e & 1, v = (e >> 1) & 1;
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    off[a] = 0;
+    off[b] = u;
+    off[c] = v;
+}
+__global__ void k_fedge_mark(int64_t n0, const uint32_t* morton, int R, RankWord* g) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 12 * n0) return;
+    const int64_t i = t / 12;
+    const int e = (int)(t - 12 * i);
+    int x, y, z, a, off[3];
+    fine_xyz(morton[i], x, y, z);
+    cell_edge(e, a, off);
+    const uint64_t bit = fedge_bit(R, x + off[0], y + off[1], z + off[2], 2 - a);
+    atomicOr(&g[bit >> 5].bits, 1u << (bit & 31));
+}
+__global__ void k_fedge_owner(int64_t n0, const uint32_t* morton, int R, const RankWord* g, uint32_t* owner) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 12 * n0) return;
+    const int64_t i = t / 12;
+    const int e = (int)(t - 12 * i);
+    int x, y, z, a, off[3];
+    fine_xyz(morton[i], x, y, z);
+    cell_edge(e, a, off);
+    const uint64_t bit = fedge_bit(R, x + off[0], y + off[1], z + off[2], 2 - a);
+    const RankWord w = g[bit >> 5];
+    const uint32_t r = w.rank + __popc(w.bits & ((1u << (bit & 31)) - 1u));
+    atomicMin(&owner[r], (uint32_t)i);
+}
+__global__ void k_fedge_emit(int64_t nwords, const RankWord* g, int R, const uint32_t* morton, const double* origin,
+                             const double* width, const uint32_t* owner, double* ends, uint32_t* own_out) {
+    const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= nwords) return;
+    const RankWord w = g[wi];
+    uint32_t bits = w.bits, r = w.rank;
+    const uint64_t D = (uint64_t)R + 1;
+    while (bits) {
+        const int k = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const uint64_t bit = (uint64_t)wi * 32u + (uint64_t)k;
+        const int slot = (int)(bit % 3u);
+        const uint64_t cell = bit / 3u;
+        const int Z = (int)(cell % D), Y = (int)((cell / D) % D), X = (int)(cell / (D * D));
+        const int a = 2 - slot;
+        const uint32_t node = owner[r];
+        int x, y, z;
+        fine_xyz(morton[node], x, y, z);
+        const int lo[3] = {X - x, Y - y, Z - z};
+        const double* o = origin + 3 * (int64_t)node;
+        const double wd = width[node];
+        for (int c = 0; c < 3; ++c) {
+            const int hi = lo[c] + (c == a ? 1 : 0);
+            ends[6 * (int64_t)r + c] = o[c] + (lo[c] ? wd : 0.0);
+            ends[6 * (int64_t)r + 3 + c] = o[c] + (hi ? wd : 0.0);
+        }
+        own_out[r] = node;
+        ++r;
+    }
+}
+}  // namespace
+
+int64_t fine_edges(its_svo s, DBuf<double>& ends, DBuf<uint32_t>& owner, hipStream_t st) {
+    const int64_t n0 = s->esum_nodes[1] - s->esum_nodes[0];
+    const uint32_t* morton = s->morton.p + s->esum_nodes[0];
+    const int R = s->R;
+    const uint64_t nbits = 3ull * ((uint64_t)R + 1) * ((uint64_t)R + 1) * ((uint64_t)R + 1);
+    const int64_t nwords = (int64_t)((nbits + 31) / 32);
+    DBuf<RankWord> g(nwords);
+    ITS_HIP(hipMemsetAsync(g.p, 0, g.bytes(), st));
+    if (n0 > 0) {
+        hipLaunchKernelGGL(k_fedge_mark, dim3(blocks_for(12 * n0, kT)), dim3(kT), 0, st, n0, morton, R, g.p);
+        ITS_LAUNCH_CHECK();
+    }
+    const int64_t n = (int64_t)rank_build(g.p, nwords, st);
+    DBuf<uint32_t> own(std::max<int64_t>(n, 1));
+    ITS_HIP(hipMemsetAsync(own.p, 0xFF, own.bytes(), st));
+    if (n0 > 0) {
+        hipLaunchKernelGGL(k_fedge_owner, dim3(blocks_for(12 * n0, kT)), dim3(kT), 0, st, n0, morton, R,
+                           (const RankWord*)g.p, own.p);
+        ITS_LAUNCH_CHECK();
+    }
+    ends.alloc(6 * std::max<int64_t>(n, 1));
+    owner.alloc(std::max<int64_t>(n, 1));
+    hipLaunchKernelGGL(k_fedge_emit, dim3(blocks_for(nwords, kT)), dim3(kT), 0, st, nwords, (const RankWord*)g.p, R,
+                       morton, (const double*)s->node_origin.p + 3 * s->esum_nodes[0],
+                       (const double*)s->node_width.p + s->esum_nodes[0], (const uint32_t*)own.p, ends.p, owner.p);
+    ITS_LAUNCH_CHECK();
+    ITS_HIP(hipStreamSynchronize(st));
+    return n;
+}
+
 void shell_points(its_svo s, const double* d_V, const uint64_t* d_edges, int64_t ne, const double* d_tris, int64_t nT,
                   DBuf<double>& pts, int64_t* n_edge, int64_t* n_face, hipStream_t st) {
     Fine S{};

@@ -187,6 +187,20 @@ public:
         return out;
     }
 
+    // fineNodeEdgeArray: level-0 node edges (lower end, upper end) and the
+    // smallest index of the level-0 nodes containing each (SVO.cu:965-1110)
+    std::vector<std::pair<std::pair<V3d, V3d>, uint32_t>> fineNodeEdgeArray() const {
+        int64_t n = 0;
+        check(its_svo_export_fine_edges(svo_, nullptr, nullptr, 0, &n));
+        std::vector<double> e(6 * (size_t)n);
+        std::vector<uint32_t> o((size_t)n);
+        if (n) check(its_svo_export_fine_edges(svo_, e.data(), o.data(), n, &n));
+        std::vector<std::pair<std::pair<V3d, V3d>, uint32_t>> out((size_t)n);
+        for (int64_t i = 0; i < n; ++i)
+            out[i] = {{V3d(e[6 * i], e[6 * i + 1], e[6 * i + 2]), V3d(e[6 * i + 3], e[6 * i + 4], e[6 * i + 5])}, o[i]};
+        return out;
+    }
+
     // cubes of every level, one OBJ per level: <dir>/svo_1.obj (finest) ..
     // svo_<treeDepth>.obj (SVO.cu:1266-1318, saveSVO)
     void saveSVO(const std::string& dir) const {

@@ -1352,6 +1352,82 @@ static void push3(pbuf* b, const double* q) {
     b->n++;
 }
 
+/* Fine node edges (determineNodeEdge, src/cuAcc/SVO.cu:965-1008): 12 edges
+ * per level-0 node, each tagged with the smallest index among the node and
+ * its neighbours sharing the edge (d_edgeSharedLUT).  Sorted by the first
+ * end, then the second end, in the fuzzy order of lessPoint, then by that
+ * index (sortEdge, SVO.cu:1033-1046); of each run of approximately equal
+ * edges (uniqueEdge, SVO.cu:1054-1060, Eigen isApprox at 1e-12) the first
+ * is kept.  Entries equal in that order come from different nodes and differ
+ * by at most an ulp; the reference's unstable sort leaves their order open,
+ * and here the owner node's own entry comes first. */
+typedef struct { double a[3], b[3]; uint32_t idx, node; } oedge;
+static int cmp_oedge(const void* x, const void* y) {
+    const oedge* p = (const oedge*)x;
+    const oedge* q = (const oedge*)y;
+    int t = -less_point(p->a, q->a);   /* lessPoint: 1 less, -1 greater, 0 equal */
+    if (t) return t;
+    t = -less_point(p->b, q->b);
+    if (t) return t;
+    if (p->idx != q->idx) return p->idx < q->idx ? -1 : 1;
+    const int po = p->node == p->idx, qo = q->node == q->idx;
+    if (po != qo) return po ? -1 : 1;
+    return p->node < q->node ? -1 : (p->node > q->node ? 1 : 0);
+}
+static int approx3(const double* a, const double* b) {
+    double d = 0, na = 0, nb = 0;
+    for (int i = 0; i < 3; ++i) { d += (a[i] - b[i]) * (a[i] - b[i]); na += a[i] * a[i]; nb += b[i] * b[i]; }
+    return d <= 1e-24 * (na < nb ? na : nb);
+}
+int64_t ora_fine_edges(const ora_svo* s, double* ends, uint32_t* owner, int64_t cap) {
+    static const int eo[12][2][3] = {
+        {{0, 0, 0}, {0, 1, 0}}, {{0, 1, 0}, {1, 1, 0}}, {{1, 0, 0}, {1, 1, 0}}, {{0, 0, 0}, {1, 0, 0}},
+        {{0, 0, 1}, {0, 1, 1}}, {{0, 1, 1}, {1, 1, 1}}, {{1, 0, 1}, {1, 1, 1}}, {{0, 0, 1}, {1, 0, 1}},
+        {{0, 0, 0}, {0, 0, 1}}, {{0, 1, 0}, {0, 1, 1}}, {{1, 1, 0}, {1, 1, 1}}, {{1, 0, 0}, {1, 0, 1}}};
+    int16_t lp[8 * 27], lc[8 * 27], vs[64], es[48];
+    ora_neighbor_luts(lp, lc, vs, es);
+    const int64_t n0 = s->depth_num_nodes[0];
+    oedge* E = (oedge*)malloc(sizeof(oedge) * 12 * (n0 > 0 ? n0 : 1));
+    for (int64_t i = 0; i < n0; ++i) {
+        const int64_t g = s->esum_nodes[0] + i;
+        const double* o = s->origin + 3 * g;
+        const double w = s->width[g];
+        for (int e = 0; e < 12; ++e) {
+            oedge* q = &E[12 * i + e];
+            for (int a = 0; a < 3; ++a) {
+                q->a[a] = o[a] + (eo[e][0][a] ? w : 0.0);
+                q->b[a] = o[a] + (eo[e][1][a] ? w : 0.0);
+            }
+            uint32_t idx = (uint32_t)g;
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t nb = s->neighbors[27 * g + es[4 * e + j]];
+                if (nb < idx) idx = nb;
+            }
+            q->idx = idx;
+            q->node = (uint32_t)g;
+        }
+    }
+    qsort(E, 12 * n0, sizeof(oedge), cmp_oedge);
+    int64_t n = 0;
+    for (int64_t k = 0; k < 12 * n0; ++k) {
+        if (k > 0) {
+            const oedge* p = &E[k - 1];
+            const oedge* q = &E[k];
+            if ((approx3(p->a, q->a) && approx3(p->b, q->b)) || (approx3(p->a, q->b) && approx3(p->b, q->a))) continue;
+        }
+        if (n < cap) {
+            if (ends) {
+                memcpy(ends + 6 * n, E[k].a, 3 * sizeof(double));
+                memcpy(ends + 6 * n + 3, E[k].b, 3 * sizeof(double));
+            }
+            if (owner) owner[n] = E[k].idx;
+        }
+        ++n;
+    }
+    free(E);
+    return n;
+}
+
 int64_t ora_intersections(const ora_svo* s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* out,
                           int64_t cap, int64_t* n_edge) {
     (void)nV;

@@ -57,6 +57,11 @@ void ora_svo_nodes(const ora_svo* s, uint32_t* morton, uint8_t* is_leaf, double*
                    uint32_t* parent, uint32_t* childs, uint32_t* neighbors);
 void ora_svo_verts(const ora_svo* s, double* xyz, uint32_t* owner);
 
+/* fine node edges with owners, in the order of the reference's
+ * fineNodeEdgeArray (src/cuAcc/SVO.cu:965-1008,1033-1060,1098-1110): 6
+ * doubles and an owner index per edge; writes up to cap, returns the count */
+int64_t ora_fine_edges(const ora_svo* s, double* ends, uint32_t* owner, int64_t cap);
+
 /* ---- hat basis (src/BSpline.hpp:5-27) ---- */
 double ora_base_function(double x, double c, double w);
 double ora_base_function3(const double c[3], double w, const double q[3]);

@@ -76,6 +76,7 @@ def lib():
             "ora_winding_number": (C.c_double, [_f64p, _i32p, C.c_int64, _f64p]),
             "ora_num_threads": (C.c_int, []),
             "ora_intersections": (C.c_int64, [vp, _f64p, C.c_int64, _i32p, C.c_int64, vp, C.c_int64, _i64p]),
+            "ora_fine_edges": (C.c_int64, [vp, vp, vp, C.c_int64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -228,6 +229,16 @@ def intersections(svo: SVO, V, F):
     lib().ora_intersections(svo._h, V.reshape(-1), len(V), F.reshape(-1), len(F), out.ctypes.data_as(C.c_void_p), n,
                             ne)
     return out[:n], int(ne[0])
+
+
+def fine_edges(svo: SVO):
+    """Level-0 node edges (fineNodeEdgeArray, src/cuAcc/SVO.cu:965-1110):
+    (n, 2, 3) end points and the owner node of each, in the reference's order."""
+    n = lib().ora_fine_edges(svo._h, None, None, 0)
+    ends = np.zeros((max(n, 1), 2, 3))
+    owner = np.zeros(max(n, 1), np.uint32)
+    lib().ora_fine_edges(svo._h, ends.ctypes.data_as(C.c_void_p), owner.ctypes.data_as(C.c_void_p), n)
+    return ends[:n], owner[:n]
 
 
 def lscg_operator(mul, tmul, invdiag, b, tol=1e-6, max_iter=None, x0=None):

@@ -89,3 +89,42 @@ def test_svo_structural_invariants(ctx):
     assert np.all(np.isclose(off_v, 0) | np.isclose(off_v, 1))
     # voxel count matches the oracle's voxeliser (cheap to check at full size)
     np.testing.assert_array_equal(gpu.voxels(), O.voxelize(c.tris, c.origin, c.width, 256))
+
+
+@pytest.mark.parametrize("name,R", [("icosphere", 16), ("bunny", 64), ("torus", 32), ("single", 8)])
+def test_fine_edges_match_oracle(ctx, name, R):
+    """Level-0 node edges (fineNodeEdgeArray, SVO.cu:965-1110): order, owner
+    and end-point bits equal to the oracle's restatement."""
+    from its_amd import _capi
+    c = case(name, R)
+    print(f"[fine edges] {c}", flush=True)
+    g = _capi.Octree(ctx, c.tris, c.origin, c.width, R)
+    s = O.SVO(c.tris, c.origin, c.width, R)
+    ends, owner = g.fine_edges()
+    ends_o, owner_o = O.fine_edges(s)
+    assert len(owner) == len(owner_o) > 0
+    np.testing.assert_array_equal(owner, owner_o)
+    np.testing.assert_array_equal(ends, ends_o)
+    g.close()
+
+
+@pytest.mark.parametrize("name,R", [("bunny", 64), ("torus", 32)])
+def test_vertex_lattice_keys(ctx, name, R):
+    """Vertex lattice keys: level d vertices sit in the d-th range of the
+    global order, ascending in (x, y, z) (the order of depthNodeVertexArray,
+    SVO.cu:1064-1096), at origin + key * (voxel width * 2^d) up to rounding."""
+    from its_amd import _capi
+    c = case(name, R)
+    g = _capi.Octree(ctx, c.tris, c.origin, c.width, R)
+    ijk, lev = g.vert_lattice()
+    xyz, _ = g.verts()
+    es = g.esum_verts()
+    for d in range(g.depth):
+        sl = slice(int(es[d]), int(es[d + 1]))
+        assert np.all(lev[sl] == d)
+        k = ijk[sl].astype(np.int64)
+        key = (k[:, 0] * (R + 1) + k[:, 1]) * (R + 1) + k[:, 2]
+        assert np.all(np.diff(key) > 0)
+        w = c.width[0] / R * 2.0 ** d
+        assert np.abs(c.origin + k * w - xyz[sl]).max() <= 1e-12
+    g.close()

@@ -387,3 +387,45 @@ def test_matrix_free_lscg_matches_assembled():
     assert st["iterations"] == F.st["iterations"]
     # the C restatement sums its dots in fixed blocks, numpy in its own order
     assert np.abs(x - F.lam).max() <= 1e-9 * np.abs(F.lam).max()
+
+
+@pytest.mark.parametrize("name,R", [("bunny", 16), ("icosphere", 16)])
+def test_fine_edges_brute_force(name, R):
+    """Fine node edges (fineNodeEdgeArray, SVO.cu:965-1110): the restatement's
+    fuzzy sort / unique / owner against integer lattice edges built directly:
+    each distinct edge of the level-0 cells once, ordered by its lower end
+    (x, then y, then z) and then z-, y-, x-edge, owned by the smallest
+    level-0 node containing it, with end points from that node's origin."""
+    from helpers import case
+    c = case(name, R)
+    s = O.SVO(c.tris, c.origin, c.width, R)
+    ends, owner = O.fine_edges(s)
+    n0 = int(s.depth_num_nodes[0])
+    best = {}
+    offs = {2: (1, 0, 0), 1: (0, 1, 0), 0: (0, 0, 1)}   # slot -> axis step (x-edge sorts last)
+    for i in range(n0):
+        m = int(s.morton[i])
+        x, y, z = [sum(((m >> (3 * b + a)) & 1) << b for b in range(10)) for a in range(3)]
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    for slot, st in offs.items():
+                        # edges of the cell: lower end on the cell, step inside the cell
+                        lo = (dx, dy, dz)
+                        hi = tuple(l + t for l, t in zip(lo, st))
+                        if max(hi) > 1:
+                            continue
+                        key = (x + dx, y + dy, z + dz, slot)
+                        if key not in best or i < best[key][0]:
+                            best[key] = (i, lo, hi)
+    keys = sorted(best)
+    assert len(keys) == len(owner)
+    exp_owner = np.array([best[k][0] for k in keys], np.uint32)
+    np.testing.assert_array_equal(owner, exp_owner)
+    exp = np.zeros((len(keys), 2, 3))
+    for r, k in enumerate(keys):
+        i, lo, hi = best[k]
+        o, w = s.node_origin[i], s.node_width[i]
+        exp[r, 0] = [o[a] + (w if lo[a] else 0.0) for a in range(3)]
+        exp[r, 1] = [o[a] + (w if hi[a] else 0.0) for a in range(3)]
+    np.testing.assert_array_equal(ends, exp)
