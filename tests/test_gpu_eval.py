@@ -154,3 +154,42 @@ def test_eval_schemes_agree(ctx, restore_tunables, dense_cells):
             else:
                 assert np.array_equal(f[:-64], ref[:-64])
     g.close()
+
+
+@pytest.mark.parametrize("name,R", [("bunny", 64), ("torus", 32)])
+def test_eval_boundary_points(ctx, name, R):
+    """Points on the box faces (open box in SUPPORT mode, closed in DENSE),
+    one binary32 step inside and outside them, and lattice points of every
+    level (cell corners, where the walk picks a cell by rounding): binary32
+    evaluation against the oracle's binary64 support sum at the same points."""
+    from its_amd import _capi
+    F = fitted(name, R)
+    c = F.case
+    g = _gpu(ctx, F)
+    rng = np.random.default_rng(3)
+    o32, e32 = c.origin.astype(np.float32), c.end.astype(np.float32)
+    pts = []
+    for a in range(3):   # faces: the bound itself and its binary32 neighbours
+        for bound in (c.origin[a], c.end[a]):
+            for v in (np.float32(bound), np.nextafter(np.float32(bound), np.float32(-np.inf)),
+                      np.nextafter(np.float32(bound), np.float32(np.inf))):
+                p = rng.uniform(o32 + 0.1, e32 - 0.1, size=(40, 3)).astype(np.float32)
+                p[:, a] = v
+                pts.append(p)
+    w = c.width[0] / R
+    for d in range(int(np.log2(R)) + 1):   # lattice points of level d
+        k = rng.integers(0, R // (1 << d) + 1, size=(200, 3))
+        pts.append((c.origin + k * w * (1 << d)).astype(np.float32))
+    P = np.concatenate(pts)
+    out_val = -2.5
+    fg = g.eval(P, c.end, _capi.EVAL_SUPPORT, out_val)
+    fr = F.svo.eval_support(F.lam, P.astype(np.float64), c.end, out_val)
+    # the open-box decision is exact: out_val on both sides at the same points
+    np.testing.assert_array_equal(fg == np.float32(out_val), fr == out_val)
+    assert np.abs(fg.astype(np.float64) - fr).max() <= TOL
+    # closed box: the face points themselves are inside
+    fd = g.eval(P, c.end, _capi.EVAL_DENSE, 7.0)
+    Pd = P.astype(np.float64)
+    inside = np.all((Pd >= c.origin - c.width) & (Pd <= c.end + c.width), axis=1)
+    assert np.all(fd[~inside] == np.float32(7.0)) and np.all(fd[inside] != np.float32(7.0))
+    g.close()
