@@ -127,9 +127,18 @@ def main():
     mc_runs = [svo.marching_cubes(origin, width, (R, R, R), 0.0)[1] for _ in range(3)]
     mc = {"res": R, "vertices": mc_runs[-1]["num_verts"],
           "seconds_lattice": float(np.median([m["seconds_eval"] for m in mc_runs])),
-          "seconds_extract": float(np.median([m["seconds_extract"] for m in mc_runs]))}
-    log(f"[bench] mc 256^3: {mc['vertices']} vertices, lattice {mc['seconds_lattice']*1e3:.2f} ms, "
-        f"extract {mc['seconds_extract']*1e3:.2f} ms")
+          "seconds_extract": float(np.median([m["seconds_extract"] for m in mc_runs])),
+          "seconds_lattice_device": float(np.median([m["seconds_eval_device"] for m in mc_runs])),
+          "seconds_extract_device": float(np.median([m["seconds_extract_device"] for m in mc_runs]))}
+    # algorithmic bytes: the (R+1)^3 binary32 lattice written and read, 4 B
+    # per voxel of case/count, 36 B per triangle of binary32 soup
+    mc_bytes = 8.0 * (R + 1) ** 3 + 4.0 * R ** 3 + 12.0 * mc["vertices"]
+    mc_dev = mc["seconds_lattice_device"] + mc["seconds_extract_device"]
+    mc["roofline"] = {"bound": "hbm", "algorithmic_bytes": mc_bytes, "achieved": mc_bytes / mc_dev / 1e9,
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": mc_bytes / mc_dev / 1e9 / HBM_PEAK_GBS}
+    log(f"[bench] mc 256^3: {mc['vertices']} vertices, lattice {mc['seconds_lattice']*1e3:.2f} ms "
+        f"(device {mc['seconds_lattice_device']*1e3:.3f}), extract {mc['seconds_extract']*1e3:.2f} ms "
+        f"(device {mc['seconds_extract_device']*1e3:.3f})")
 
     # queries resident in HBM
     g = torch.Generator(device=dev)
@@ -158,6 +167,41 @@ def main():
     elapsed = time.perf_counter() - t0
     avg_ms = e0.elapsed_time(e1) / args.steps
     value = N_QUERIES * args.steps / elapsed
+
+    # end to end with the queries in host memory (outside the timed region):
+    # pinned SoA buffers copied in and out on the launch stream around the
+    # same launch, and the host entry point (pageable binary32 xyz in
+    # interleaved order, f out)
+    qh = q.cpu().pin_memory()
+    fh = torch.empty(N_QUERIES, dtype=torch.float32).pin_memory()
+    qd = torch.empty_like(q)
+
+    def step_pinned():
+        qd.copy_(qh, non_blocking=True)
+        svo.eval_device(qd[0].data_ptr(), qd[1].data_ptr(), qd[2].data_ptr(), N_QUERIES, end, out.data_ptr(),
+                        _capi.EVAL_SUPPORT, 0.0, stream=stream.cuda_stream)
+        fh.copy_(out, non_blocking=True)
+
+    def wall(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    t_pinned = wall(step_pinned)
+    pinned_ok = bool(torch.equal(fh, out.cpu()))
+    q_aos = qh.numpy().T.copy()
+    t_host = wall(lambda: svo.eval(q_aos, end))
+    e2e = {"pinned_soa_ms": t_pinned * 1e3, "pinned_evals_per_s": N_QUERIES / t_pinned,
+           "host_entry_ms": t_host * 1e3, "host_entry_evals_per_s": N_QUERIES / t_host,
+           "pinned_equals_resident": pinned_ok}
+    log(f"[bench] end to end: pinned {t_pinned * 1e3:.2f} ms, host entry point {t_host * 1e3:.2f} ms")
+    del qh, fh, qd, q_aos
 
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib * (1 << 18), device=dev, dtype=torch.float32)
@@ -225,6 +269,7 @@ def main():
             "nodes": svo.num_nodes,
         },
         "eval_kernel_ms": {"mean": avg_ms},
+        "eval_end_to_end": e2e,
         "lscg_cold": {"seconds": t_lscg, "iterations": st["iterations"], "normal_rel": st["normal_rel"],
                       "residual_rel": st["residual_rel"]},
         "lscg_warm": warm,

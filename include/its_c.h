@@ -186,8 +186,13 @@ int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, 
 typedef struct {
     int64_t num_verts;        /* allTriVertices */
     int64_t num_valid_voxels; /* nValidVoxels */
-    double seconds_eval;      /* lattice evaluation */
-    double seconds_extract;   /* classify + scan + emit */
+    double seconds_eval;      /* lattice evaluation (host clock) */
+    double seconds_extract;   /* count, scan and emission, with the copy of the
+                                 soup to the caller's buffer (host clock) */
+    double seconds_eval_device;      /* lattice evaluation, between events on the
+                                        context stream */
+    double seconds_extract_device;   /* count, scan and emission on the stream,
+                                        without the copy to the host */
 } its_mc_stats;
 int its_mc_extract(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
                    double iso, float* tri_xyz, int64_t capacity, its_mc_stats* stats);

@@ -69,6 +69,8 @@ class McStats(C.Structure):
         ("num_valid_voxels", C.c_int64),
         ("seconds_eval", C.c_double),
         ("seconds_extract", C.c_double),
+        ("seconds_eval_device", C.c_double),
+        ("seconds_extract_device", C.c_double),
     ]
 
 
@@ -473,7 +475,9 @@ class Octree:
         check(lib().its_mc_extract(self.ctx.handle, self._h, _f64(origin), _f64(width), res, iso,
                                    tri.ctypes.data_as(vp), n, C.byref(st)))
         return tri[:n], {"num_verts": st.num_verts, "num_valid_voxels": st.num_valid_voxels,
-                         "seconds_eval": st.seconds_eval, "seconds_extract": st.seconds_extract}
+                         "seconds_eval": st.seconds_eval, "seconds_extract": st.seconds_extract,
+                         "seconds_eval_device": st.seconds_eval_device,
+                         "seconds_extract_device": st.seconds_extract_device}
 
     def mc_lattice(self, origin, width, res):
         res = np.asarray(res, np.int32)

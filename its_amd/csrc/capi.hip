@@ -85,6 +85,7 @@ int its_ctx_create(int device, its_ctx* out) {
     c->own_stream = true;
     ITS_HIP(hipEventCreate(&c->ev0));
     ITS_HIP(hipEventCreate(&c->ev1));
+    for (hipEvent_t& e : c->mc_ev) ITS_HIP(hipEventCreate(&e));
     ITS_HIP(hipGetDeviceProperties(&c->prop, device));
     *out = c;
     ITS_API_END
@@ -98,6 +99,8 @@ int its_ctx_destroy(its_ctx ctx) {
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->mc_ev)
+        if (e) (void)hipEventDestroy(e);
     delete ctx;
     ITS_API_END
 }

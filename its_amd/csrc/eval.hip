@@ -512,25 +512,61 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
 
 // One level of the dense prolongation cascade on the full lattice:
 // g_d(X) = lambda_d(X) (0 where X is not a vertex) + trilinear interpolation
-// of g_{d+1} at X/2.  binary64, x-fastest grids.
-__global__ void k_dense_level(int D, const double* gc, int Dcoarse, const RankWord* lat, int64_t voff,
-                              const double* lambda, double* g) {
-    int64_t n = (int64_t)D * D * D;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int X = (int)(i % D), Y = (int)((i / D) % D), Z = (int)(i / ((int64_t)D * D));
-        double v = 0.0;
-        if (gc) {
-            int xs[2] = {X >> 1, (X + 1) >> 1}, ys[2] = {Y >> 1, (Y + 1) >> 1}, zs[2] = {Z >> 1, (Z + 1) >> 1};
-            int nx = (X & 1) ? 2 : 1, ny = (Y & 1) ? 2 : 1, nz = (Z & 1) ? 2 : 1;
-            double s = 0.0;
-            for (int c = 0; c < nz; ++c)
-                for (int b = 0; b < ny; ++b)
-                    for (int a = 0; a < nx; ++a) s += gc[((int64_t)zs[c] * Dcoarse + ys[b]) * Dcoarse + xs[a]];
-            v = s / (double)(nx * ny * nz);
+// of g_{d+1} at X/2, summed in binary64; x-fastest grids.  A thread walks a
+// run of zrun (<= 32) points along z at one (X, Y), so the writes of a wave
+// (consecutive X) stay contiguous.  The vertex bitset is keyed z-fastest: the
+// run's keys lie in at most two of its words, both loaded before the walk.
+// Consecutive points share coarse planes, so each plane's bilinear sum is
+// formed once.  OutT = float rounds the values of the last level as they are
+// written.
+constexpr int kZRunMax = 32;
+template <typename OutT>
+__global__ void k_dense_level(int D, int zrun, const double* __restrict__ gc, int Dcoarse,
+                              const RankWord* __restrict__ lat, int64_t voff, const double* __restrict__ lambda,
+                              OutT* __restrict__ g) {
+    const int64_t nzr = (D + zrun - 1) / zrun;
+    const int64_t nt = (int64_t)D * D * nzr;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
+        const int X = (int)(t % D);
+        const int64_t rest = t / D;
+        const int Y = (int)(rest % D), Z0 = (int)(rest / D) * zrun;
+        const int Z1 = min(Z0 + zrun, D);
+        const int x0 = X >> 1, x1 = (X + 1) >> 1, y0 = Y >> 1, y1 = (Y + 1) >> 1;
+        const bool two_x = X & 1, two_y = Y & 1;
+        const double nxy = (double)((two_x ? 2 : 1) * (two_y ? 2 : 1));
+        const uint32_t key0 = (uint32_t)(((int64_t)X * D + Y) * D);
+        const uint32_t wlo = (key0 + (uint32_t)Z0) >> 5;
+        const RankWord w0 = lat[wlo], w1 = lat[(key0 + (uint32_t)Z1 - 1u) >> 5];
+        // bilinear sum (not yet divided) of the coarse values on plane p
+        auto plane = [&](int p) {
+            const double* r0 = gc + ((int64_t)p * Dcoarse + y0) * Dcoarse;
+            double s = r0[x0];
+            if (two_x) s += r0[x1];
+            if (two_y) {
+                const double* r1 = gc + ((int64_t)p * Dcoarse + y1) * Dcoarse;
+                s += r1[x0];
+                if (two_x) s += r1[x1];
+            }
+            return s;
+        };
+        double cur = gc ? plane(Z0 >> 1) : 0.0;
+        for (int Z = Z0; Z < Z1; ++Z) {
+            double v = 0.0;
+            if (gc) {
+                if (Z & 1) {
+                    const double nxt = plane((Z + 1) >> 1);
+                    v = (cur + nxt) / (2.0 * nxy);
+                    cur = nxt;
+                } else {
+                    v = cur / nxy;
+                }
+            }
+            const uint32_t key = key0 + (uint32_t)Z;
+            const RankWord w = (key >> 5) == wlo ? w0 : w1;
+            const uint32_t bit = 1u << (key & 31);
+            if (w.bits & bit) v += lambda[voff + w.rank + __popc(w.bits & (bit - 1u))];
+            g[((int64_t)Z * D + Y) * D + X] = (OutT)v;
         }
-        uint32_t r = bit_find(lat, (uint32_t)(((int64_t)X * D + Y) * D + Z));
-        if (r != ITS_NONE) v += lambda[voff + r];
-        g[i] = v;
     }
 }
 // lattice values (x fastest, D = Rc + 1 per axis) -> one 64-byte block per
@@ -595,10 +631,6 @@ __global__ void k_emit_codes(const RankWord* g, int64_t nwords, uint32_t* out) {
         out[r++] = (uint32_t)(w * 32 + i);
         b &= b - 1;
     }
-}
-__global__ void k_to_f32(int64_t n, const double* a, float* b) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = (float)a[i];
 }
 
 EvalParams make_params(its_svo s, const double box_end[3], int mode, float out_val) {
@@ -758,36 +790,56 @@ void field_build(its_svo s) {
     s->field_ready = true;
 }
 
-// f at every lattice point of level `stop` (values of levels >= stop only
-// when stop > 0), binary32, x fastest
-void dense_cascade64(its_svo s, int stop, DBuf<double>& out) {
-    hipStream_t st = s->ctx->stream;
-    DBuf<double> cur;
-    int Dprev = 0;
-    for (int d = s->depth - 1; d >= stop; --d) {
+namespace {
+// runs of 32 points on large levels (fewest bitset words per point); shorter
+// runs on small levels, which would otherwise have too few threads
+template <typename OutT>
+void launch_dense_level(its_svo s, int d, const double* coarse, int Dcoarse, OutT* g, hipStream_t st) {
+    const int D = s->lv[d].R + 1;
+    int zrun = kZRunMax;
+    while (zrun > 4 && (int64_t)D * D * ((D + zrun - 1) / zrun) < (int64_t)1 << 19) zrun >>= 1;
+    const int64_t nt = (int64_t)D * D * ((D + zrun - 1) / zrun);
+    const unsigned nb = (unsigned)std::min<int64_t>(blocks_for(nt, kT), 65536);
+    hipLaunchKernelGGL((k_dense_level<OutT>), dim3(nb), dim3(kT), 0, st, D, zrun, coarse, Dcoarse,
+                       (const RankWord*)s->lv[d].lattice.p, s->esum_verts[d], (const double*)s->lambda64.p, g);
+    ITS_LAUNCH_CHECK();
+}
+
+// binary64 cascade from the root level down to level stop + 1, in the octree's
+// two grow-only cascade buffers; returns the buffer holding level stop + 1
+// (nullptr when stop is the root level)
+const double* cascade_above(its_svo s, int stop, hipStream_t st) {
+    const double* prev = nullptr;
+    int Dprev = 0, k = 0;
+    for (int d = s->depth - 1; d > stop; --d, k ^= 1) {
         const int D = s->lv[d].R + 1;
-        const int64_t n = (int64_t)D * D * D;
-        cur.alloc(n);
-        unsigned nb = (unsigned)std::min<int64_t>(blocks_for(n, kT), 65536);
-        hipLaunchKernelGGL(k_dense_level, dim3(nb), dim3(kT), 0, st, D, d == s->depth - 1 ? nullptr : out.p, Dprev,
-                           s->lv[d].lattice.p, s->esum_verts[d], s->lambda64.p, cur.p);
-        ITS_LAUNCH_CHECK();
-        out = std::move(cur);
+        s->casc[k].ensure((size_t)D * D * D);
+        launch_dense_level<double>(s, d, prev, Dprev, s->casc[k].p, st);
+        prev = s->casc[k].p;
         Dprev = D;
     }
+    return prev;
+}
+}  // namespace
+
+// f at every lattice point of level `stop` (values of levels >= stop only
+// when stop > 0), x fastest; binary64 throughout, or rounded to binary32 as
+// the last level is written
+void dense_cascade64(its_svo s, int stop, DBuf<double>& out) {
+    hipStream_t st = s->ctx->stream;
+    const double* coarse = cascade_above(s, stop, st);
+    const int D = s->lv[stop].R + 1;
+    out.ensure((size_t)D * D * D);
+    launch_dense_level<double>(s, stop, coarse, stop + 1 < s->depth ? s->lv[stop + 1].R + 1 : 0, out.p, st);
     ITS_HIP(hipStreamSynchronize(st));
 }
 
 void dense_cascade(its_svo s, int stop, DBuf<float>& out) {
     hipStream_t st = s->ctx->stream;
-    DBuf<double> lat;
-    dense_cascade64(s, stop, lat);
+    const double* coarse = cascade_above(s, stop, st);
     const int D = s->lv[stop].R + 1;
-    const int64_t n = (int64_t)D * D * D;
-    out.ensure(n);
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<int64_t>(blocks_for(n, kT), 65536)), dim3(kT), 0, st, n,
-                       lat.p, out.p);
-    ITS_LAUNCH_CHECK();
+    out.ensure((size_t)D * D * D);
+    launch_dense_level<float>(s, stop, coarse, stop + 1 < s->depth ? s->lv[stop + 1].R + 1 : 0, out.p, st);
     ITS_HIP(hipStreamSynchronize(st));
 }
 

@@ -35,7 +35,8 @@ struct its_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;            // its_timer_start / its_timer_stop
+    hipEvent_t mc_ev[3] = {nullptr, nullptr, nullptr};   // marching-cubes phase times
     hipDeviceProp_t prop{};
 };
 
@@ -72,6 +73,12 @@ struct its_svo_s {
     its::DBuf<float4> dense_blocks;   // per dense_level cell: corner values of levels >= dense_level,
                                       // child link in the 5 low bits of each
     int dense_level = 0;
+    // grow-only work space: two binary64 levels of the dense cascade, and the
+    // marching-cubes lattice, soup and per-wave vertex counts
+    its::DBuf<double> casc[2];
+    its::DBuf<float> mc_lat, mc_soup;
+    its::DBuf<uint64_t> mc_wave;
+    its::DBuf<unsigned long long> mc_valid;
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};
 };

@@ -6,10 +6,14 @@
 //
 // The reference evaluates f at 8 corners per voxel.  Each corner is a brute
 // force sum over all n basis functions, batched over 1024 streams.  Here f is
-// evaluated once per lattice point ((res+1)^3 points) with the octree field
-// walk.  A classify pass writes the cube case and vertex count, an exclusive
-// scan gives output offsets, and an emit pass writes the triangle soup.  The
-// soup order is the reference's: voxels x-fastest, triangles in table order.
+// evaluated once per lattice point ((res+1)^3 points), by the dense cascade on
+// the octree's own lattice or by the octree field walk elsewhere.  Extraction
+// is two passes of the same walk, each wave over whole rows of voxels: a count
+// pass writes one vertex count per wave, an exclusive scan of those gives each
+// wave its output offset, and an emit pass re-derives the cube cases and
+// places each voxel's triangles with a wave prefix sum.  No per-voxel arrays
+// are written.  The soup order is the reference's: voxels x-fastest,
+// triangles in table order.
 // The 1e-6 snaps of vertexLerp (MarchingCubes.cu:72-90) are kept, and so is
 // the lerp fma(t, v1, fma(-t, v0, v0)).
 #include <algorithm>
@@ -24,7 +28,7 @@ void eval_lattice(its_svo s, const double o[3], const double vs[3], const int64_
 
 namespace {
 constexpr int kT = 256;
-__constant__ signed char c_tri[256][16];
+__constant__ __align__(16) signed char c_tri[256][16];
 __constant__ unsigned char c_nv[256];
 
 // reference corner order (MarchingCubes.cu:142-149): (x,y,z) offsets
@@ -32,34 +36,50 @@ __constant__ unsigned char c_cx[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __constant__ unsigned char c_cy[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 __constant__ unsigned char c_cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
-struct McDims {
+// The voxel grid as rows along x in the reference's voxel order (x fastest,
+// then y, then z): row r = j + ry * k.  Each wave walks rows_per_wave
+// consecutive rows, 64 voxels of a row at a time, in both passes.
+struct McGrid {
     int rx, ry, rz;
-    int64_t lx, ly;   // lattice dims in x, y (res+1)
+    int64_t lx, ly;   // lattice points per axis in x, y (res + 1)
+    int64_t nrows;    // ry * rz
+    int64_t nwaves;   // ceil(nrows / rows_per_wave)
+    int rows_per_wave;
     float iso;
 };
 
-__device__ inline float corner_val(const float* lat, const McDims& D, int64_t i, int64_t j, int64_t k, int c) {
-    return lat[((k + c_cz[c]) * D.ly + (j + c_cy[c])) * D.lx + (i + c_cx[c])];
+// cube case of voxel (i, j, k); base = lattice index of point (0, j, k)
+__device__ inline unsigned mc_case(const float* lat, const McGrid& G, int64_t base, int i) {
+    unsigned c = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        c |= (unsigned)(lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]] < G.iso) << q;
+    return c;
 }
 
-__global__ void k_classify(int64_t nvox, const float* lat, McDims D, uint8_t* cube, uint32_t* cnt,
-                           unsigned long long* nvalid) {
-    __shared__ unsigned int blk_valid;
-    if (threadIdx.x == 0) blk_valid = 0;
-    __syncthreads();
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nvox) {
-        int64_t i = v % D.rx, j = (v / D.rx) % D.ry, k = v / ((int64_t)D.rx * D.ry);
-        unsigned c = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) c |= (unsigned)(corner_val(lat, D, i, j, k, q) < D.iso) << q;
-        cube[v] = (uint8_t)c;
-        unsigned n = c_nv[c];
-        cnt[v] = n;
-        if (n) atomicAdd(&blk_valid, 1u);
+// pass 1: vertices per wave, and the number of voxels with triangles
+__global__ __launch_bounds__(kT) void k_mc_count(const float* lat, McGrid G, uint64_t* wave_verts,
+                                                 unsigned long long* nvalid) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+    if (w >= G.nwaves) return;   // whole waves only
+    const int64_t r0 = w * G.rows_per_wave, r1 = min(r0 + G.rows_per_wave, G.nrows);
+    uint32_t nv = 0, nval = 0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t j = r % G.ry, k = r / G.ry;
+        const int64_t base = (k * G.ly + j) * G.lx;
+        for (int i = lane; i < G.rx; i += 64) {
+            const unsigned n = c_nv[mc_case(lat, G, base, i)];
+            nv += n;
+            nval += n != 0;
+        }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && blk_valid) atomicAdd(nvalid, (unsigned long long)blk_valid);
+    nv = wave_sum(nv);
+    nval = wave_sum(nval);
+    if (lane == 0) {
+        wave_verts[w] = nv;
+        if (nval) atomicAdd(nvalid, (unsigned long long)nval);
+    }
 }
 
 struct P3 {
@@ -73,37 +93,75 @@ __device__ inline P3 vlerp(P3 p0, P3 p1, double s0, double s1, double iso) {
     return P3{fma(t, p1.x, fma(-t, p0.x, p0.x)), fma(t, p1.y, fma(-t, p0.y, p0.y)), fma(t, p1.z, fma(-t, p0.z, p0.z))};
 }
 
-__constant__ unsigned char c_e0[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
-__constant__ unsigned char c_e1[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
-
-__global__ void k_emit(int64_t nvox, const float* lat, McDims D, double ox, double oy, double oz, double vx, double vy,
-                       double vz, const uint8_t* cube, const uint32_t* offs, uint64_t max_verts, float* out) {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
-    unsigned c = cube[v];
-    unsigned nv = c_nv[c];
-    if (!nv) return;
-    int64_t i = v % D.rx, j = (v / D.rx) % D.ry, k = v / ((int64_t)D.rx * D.ry);
-    // voxelToMeshKernel: voxelPos = shift * voxelSize + origin
-    P3 vp{(double)i * vx + ox, (double)j * vy + oy, (double)k * vz + oz};
-    P3 cr[8];
-    double sv[8];
+// The corner tables above as bit masks (bit q = offset of corner q), and the
+// reference's edge numbering: edges 0-3 join corners e and (e+1) mod 4 on the
+// z = 0 face, 4-7 the same on z = 1, 8-11 join corner e-8 to e-4.  Computed
+// in registers, so no per-lane table lookups or indexed private arrays.
+constexpr unsigned kCornerX = 0xCC, kCornerY = 0x66, kCornerZ = 0xF0;
+__device__ inline int edge_a(int e) { return e < 8 ? e : e - 8; }
+__device__ inline int edge_b(int e) { return e < 8 ? ((e & 4) | ((e + 1) & 3)) : e - 4; }
+__device__ inline double pick8(const double (&v)[8], int q) {
+    double r = v[0];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        cr[q] = P3{vp.x + (c_cx[q] ? vx : 0.0), vp.y + (c_cy[q] ? vy : 0.0), vp.z + (c_cz[q] ? vz : 0.0)};
-        sv[q] = (double)corner_val(lat, D, i, j, k, q);
-    }
-    const double iso = (double)D.iso;
-    uint64_t base = offs[v];
-    for (unsigned t = 0; t < nv; t += 3) {
-        uint64_t pos = base + t;
-        if (pos >= max_verts - 3) break;
-        for (int q = 0; q < 3; ++q) {
-            int e = c_tri[c][t + q];
-            P3 p = vlerp(cr[c_e0[e]], cr[c_e1[e]], sv[c_e0[e]], sv[c_e1[e]], iso);
-            out[3 * (pos + q) + 0] = (float)p.x;
-            out[3 * (pos + q) + 1] = (float)p.y;
-            out[3 * (pos + q) + 2] = (float)p.z;
+    for (int k = 1; k < 8; ++k) r = q == k ? v[k] : r;
+    return r;
+}
+__device__ inline P3 corner_pos(const P3& vp, double vx, double vy, double vz, int q) {
+    // voxelToMeshKernel: corner = voxelPos + offset * voxelSize
+    return P3{vp.x + (((kCornerX >> q) & 1) ? vx : 0.0), vp.y + (((kCornerY >> q) & 1) ? vy : 0.0),
+              vp.z + (((kCornerZ >> q) & 1) ? vz : 0.0)};
+}
+
+// pass 2: the same walk; a wave prefix sum over the 64 voxels of a step places
+// their triangles after those of all earlier voxels (wave_off: exclusive scan
+// of pass 1)
+__global__ __launch_bounds__(kT) void k_mc_emit(const float* lat, McGrid G, double ox, double oy, double oz,
+                                                double vx, double vy, double vz, const uint64_t* wave_off,
+                                                uint64_t max_verts, float* out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+    if (w >= G.nwaves) return;
+    const int64_t r0 = w * G.rows_per_wave, r1 = min(r0 + G.rows_per_wave, G.nrows);
+    uint64_t at = wave_off[w];
+    const double iso = (double)G.iso;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t j = r % G.ry, k = r / G.ry;
+        const int64_t base = (k * G.ly + j) * G.lx;
+        for (int i0 = 0; i0 < G.rx; i0 += 64) {
+            const int i = i0 + lane;
+            unsigned c = 0, nv = 0;
+            if (i < G.rx) {
+                c = mc_case(lat, G, base, i);
+                nv = c_nv[c];
+            }
+            if (__ballot(nv != 0) == 0) continue;
+            const uint32_t incl = wave_incl_scan(nv);
+            const uint32_t step = __shfl(incl, 63, 64);
+            if (nv) {
+                // voxelToMeshKernel: voxelPos = shift * voxelSize + origin
+                const P3 vp{(double)i * vx + ox, (double)j * vy + oy, (double)k * vz + oz};
+                double sv[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sv[q] = (double)lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]];
+                const uint4 row = *reinterpret_cast<const uint4*>(&c_tri[c][0]);
+                const uint64_t first = at + incl - nv;
+                for (unsigned t = 0; t < nv; t += 3) {
+                    const uint64_t pos = first + t;
+                    if (pos >= max_verts - 3) break;
+                    for (unsigned q = 0; q < 3; ++q) {
+                        const unsigned m = t + q;
+                        const unsigned word = (m >> 2) == 0 ? row.x : (m >> 2) == 1 ? row.y : (m >> 2) == 2 ? row.z : row.w;
+                        const int e = (int)(signed char)(word >> (8 * (m & 3)));
+                        const int qa = edge_a(e), qb = edge_b(e);
+                        const P3 p = vlerp(corner_pos(vp, vx, vy, vz, qa), corner_pos(vp, vx, vy, vz, qb), pick8(sv, qa),
+                                           pick8(sv, qb), iso);
+                        out[3 * (pos + q) + 0] = (float)p.x;
+                        out[3 * (pos + q) + 1] = (float)p.y;
+                        out[3 * (pos + q) + 2] = (float)p.z;
+                    }
+                }
+            }
+            at += step;
         }
     }
 }
@@ -152,50 +210,71 @@ void mc_lattice(its_svo s, const double origin[3], const double width[3], const 
     hipStream_t st = s->ctx->stream;
     int64_t dims[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
     int64_t np = dims[0] * dims[1] * dims[2];
-    DBuf<float> lat;
-    lattice_values(s, origin, width, res, lat, st);
-    ITS_HIP(hipMemcpyAsync(h_vals, lat.p, np * sizeof(float), hipMemcpyDeviceToHost, st));
+    lattice_values(s, origin, width, res, s->mc_lat, st);
+    ITS_HIP(hipMemcpyAsync(h_vals, s->mc_lat.p, np * sizeof(float), hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
 }
 
 void mc_extract(its_svo s, const double origin[3], const double width[3], const int res[3], double iso, float* h_tri,
                 int64_t capacity, its_mc_stats* stats) {
     hipStream_t st = s->ctx->stream;
+    hipEvent_t* ev = s->ctx->mc_ev;
     upload_tables();
     ITS_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, ITS_E_INVALID, "resolution must be positive");
     its_mc_stats S{};
     auto t0 = std::chrono::steady_clock::now();
     double vs[3] = {width[0] / res[0], width[1] / res[1], width[2] / res[2]};   // MarchingCubes.cu:682
-    int64_t dims[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
     int64_t nvox = (int64_t)res[0] * res[1] * res[2];
-    DBuf<float> lat;
-    lattice_values(s, origin, width, res, lat, st);
+    ITS_HIP(hipEventRecord(ev[0], st));
+    lattice_values(s, origin, width, res, s->mc_lat, st);
+    ITS_HIP(hipEventRecord(ev[1], st));
     ITS_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
-    McDims D{res[0], res[1], res[2], dims[0], dims[1], (float)iso};
-    DBuf<uint8_t> cube(nvox);
-    DBuf<uint32_t> cnt(nvox);
-    DBuf<unsigned long long> nvalid(1);
-    ITS_HIP(hipMemsetAsync(nvalid.p, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_classify, dim3(blocks_for(nvox, kT)), dim3(kT), 0, st, nvox, lat.p, D, cube.p, cnt.p,
-                       nvalid.p);
+    McGrid G{};
+    G.rx = res[0];
+    G.ry = res[1];
+    G.rz = res[2];
+    G.lx = res[0] + 1;
+    G.ly = res[1] + 1;
+    G.nrows = (int64_t)res[1] * res[2];
+    // whole rows per wave, about 32 waves per CU over the grid
+    const int64_t target = 32 * (int64_t)std::max(1, s->ctx->prop.multiProcessorCount);
+    G.rows_per_wave = (int)std::max<int64_t>(1, (G.nrows + target - 1) / target);
+    G.nwaves = (G.nrows + G.rows_per_wave - 1) / G.rows_per_wave;
+    G.iso = (float)iso;
+    const unsigned nb = (unsigned)((G.nwaves * 64 + kT - 1) / kT);
+    s->mc_wave.ensure(G.nwaves);
+    s->mc_valid.ensure(1);
+    ITS_HIP(hipMemsetAsync(s->mc_valid.p, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_mc_count, dim3(nb), dim3(kT), 0, st, (const float*)s->mc_lat.p, G, s->mc_wave.p,
+                       s->mc_valid.p);
     ITS_LAUNCH_CHECK();
-    uint64_t total = scan_exclusive_u32(cnt.p, cnt.p, nvox, st);
+    const uint64_t total = scan_exclusive_u64(s->mc_wave.p, s->mc_wave.p, G.nwaves, st);
     S.num_verts = (int64_t)total;
-    S.num_valid_voxels = (int64_t)read1(nvalid.p, st);
-    uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
+    const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
     if (h_tri && total) {
         ITS_REQUIRE((int64_t)total <= capacity, ITS_E_RANGE, "triangle buffer too small");
-        DBuf<float> out(3 * total);
-        hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvox, kT)), dim3(kT), 0, st, nvox, lat.p, D, origin[0], origin[1],
-                           origin[2], vs[0], vs[1], vs[2], cube.p, cnt.p, max_verts, out.p);
+        s->mc_soup.ensure(3 * total);
+        hipLaunchKernelGGL(k_mc_emit, dim3(nb), dim3(kT), 0, st, (const float*)s->mc_lat.p, G, origin[0], origin[1],
+                           origin[2], vs[0], vs[1], vs[2], (const uint64_t*)s->mc_wave.p, max_verts, s->mc_soup.p);
         ITS_LAUNCH_CHECK();
-        ITS_HIP(hipMemcpyAsync(h_tri, out.p, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipEventRecord(ev[2], st));
+        ITS_HIP(hipMemcpyAsync(h_tri, s->mc_soup.p, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
+    } else {
+        ITS_HIP(hipEventRecord(ev[2], st));
     }
+    unsigned long long nvalid = 0;
+    ITS_HIP(hipMemcpyAsync(&nvalid, s->mc_valid.p, sizeof nvalid, hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
+    S.num_valid_voxels = (int64_t)nvalid;
     auto t2 = std::chrono::steady_clock::now();
+    float ms_lat = 0.f, ms_ext = 0.f;
+    ITS_HIP(hipEventElapsedTime(&ms_lat, ev[0], ev[1]));
+    ITS_HIP(hipEventElapsedTime(&ms_ext, ev[1], ev[2]));
     S.seconds_eval = std::chrono::duration<double>(t1 - t0).count();
     S.seconds_extract = std::chrono::duration<double>(t2 - t1).count();
+    S.seconds_eval_device = 1e-3 * ms_lat;
+    S.seconds_extract_device = 1e-3 * ms_ext;
     if (stats) *stats = S;
 }
 
