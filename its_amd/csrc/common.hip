@@ -1,6 +1,8 @@
 // Device-wide scans (three-phase: per-block local scan, scan of block sums,
 // add back) and the rank directory of bitsets.  wave64 shuffles do the
 // intra-wave part; one block covers 2048 items.
+#include <vector>
+
 #include "common.hpp"
 
 namespace its {
@@ -72,25 +74,35 @@ struct RankLoad {
     __device__ uint32_t operator()(int64_t i) const { return g[i].rank; }
 };
 
+// Launches the scan levels (block scans, the scan of the block sums, add-back)
+// without waiting; returns the device address of the grand total.  The
+// per-level block-sum buffers stay alive in `keep` until the caller syncs.
+template <class T, class Load, class Store, class AddLoad>
+const T* scan_launch(Load load, Store store, AddLoad addload, int64_t n, hipStream_t s, std::vector<DBuf<T>>& keep) {
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    keep.emplace_back(nb);
+    T* sums = keep.back().p;
+    hipLaunchKernelGGL((k_scan_local<T, Load, Store>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, load, store, sums,
+                       n);
+    ITS_LAUNCH_CHECK();
+    if (nb == 1) return sums;   // the single block's total
+    const T* total = scan_launch<T>(ArrLoad<T>{sums}, ArrStore<T>{sums}, ArrLoad<T>{sums}, nb, s, keep);
+    hipLaunchKernelGGL((k_scan_add<T, AddLoad, Store>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, addload, store,
+                       (const T*)sums, n);
+    ITS_LAUNCH_CHECK();
+    return total;
+}
+
 template <class T, class Load, class Store, class AddLoad>
 uint64_t scan_impl(Load load, Store store, AddLoad addload, int64_t n, hipStream_t s) {
     if (n <= 0) return 0;
-    int64_t nb = (n + kScanTile - 1) / kScanTile;
-    DBuf<T> sums(nb);
-    hipLaunchKernelGGL((k_scan_local<T, Load, Store>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, load, store, sums.p,
-                       n);
-    ITS_LAUNCH_CHECK();
-    uint64_t total;
-    if (nb == 1) {
-        total = (uint64_t)read1(sums.p, s);
-    } else {
-        total = scan_impl<T>(ArrLoad<T>{sums.p}, ArrStore<T>{sums.p}, ArrLoad<T>{sums.p}, nb, s);
-        hipLaunchKernelGGL((k_scan_add<T, AddLoad, Store>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, addload, store,
-                           sums.p, n);
-        ITS_LAUNCH_CHECK();
-    }
+    std::vector<DBuf<T>> keep;
+    keep.reserve(8);
+    const T* d_total = scan_launch<T>(load, store, addload, n, s, keep);
+    T total;
+    ITS_HIP(hipMemcpyAsync(&total, d_total, sizeof(T), hipMemcpyDeviceToHost, s));
     ITS_HIP(hipStreamSynchronize(s));
-    return total;
+    return (uint64_t)total;
 }
 }  // namespace
 

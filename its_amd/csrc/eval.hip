@@ -569,6 +569,51 @@ __global__ void k_dense_level(int D, int zrun, const double* __restrict__ gc, in
         }
     }
 }
+// The small top levels of the cascade (at most kTopD points per axis) in one
+// workgroup, level after level with a barrier between them: one launch in
+// place of one per level.  Same arithmetic per point as k_dense_level.
+constexpr int kTopD = 17;
+constexpr int kTopThreads = 1024;
+struct TopLevels {
+    int n;                          // levels, coarsest first
+    int D[kMaxLevels];              // points per axis
+    const RankWord* lat[kMaxLevels];
+    int64_t voff[kMaxLevels];
+    double* out[kMaxLevels];
+};
+__global__ __launch_bounds__(kTopThreads) void k_dense_top(TopLevels T, const double* __restrict__ lambda) {
+    for (int l = 0; l < T.n; ++l) {
+        const int D = T.D[l], Dc = l ? T.D[l - 1] : 0;
+        const double* gc = l ? T.out[l - 1] : nullptr;
+        const int n = D * D * D;
+        for (int t = threadIdx.x; t < n; t += kTopThreads) {
+            const int X = t % D, Y = (t / D) % D, Z = t / (D * D);
+            double v = 0.0;
+            if (gc) {
+                const int xs[2] = {X >> 1, (X + 1) >> 1}, ys[2] = {Y >> 1, (Y + 1) >> 1}, zs[2] = {Z >> 1, (Z + 1) >> 1};
+                const bool tx = X & 1, ty = Y & 1, tz = Z & 1;
+                const double nxy = (double)((tx ? 2 : 1) * (ty ? 2 : 1));
+                auto plane = [&](int p) {
+                    const double* r0 = gc + ((int64_t)p * Dc + ys[0]) * Dc;
+                    double s = r0[xs[0]];
+                    if (tx) s += r0[xs[1]];
+                    if (ty) {
+                        const double* r1 = gc + ((int64_t)p * Dc + ys[1]) * Dc;
+                        s += r1[xs[0]];
+                        if (tx) s += r1[xs[1]];
+                    }
+                    return s;
+                };
+                v = tz ? (plane(zs[0]) + plane(zs[1])) / (2.0 * nxy) : plane(zs[0]) / nxy;
+            }
+            const uint32_t r = bit_find(T.lat[l], (uint32_t)(((int64_t)X * D + Y) * D + Z));
+            if (r != ITS_NONE) v += lambda[T.voff[l] + r];
+            T.out[l][t] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // lattice values (x fastest, D = Rc + 1 per axis) -> one 64-byte block per
 // cell: 8 corner values (index x | y<<1 | z<<2), child base, child mask
 __global__ void k_cell_blocks(int Rc, const float* lat, float* blocks) {
@@ -810,10 +855,31 @@ void launch_dense_level(its_svo s, int d, const double* coarse, int Dcoarse, Out
 // (nullptr when stop is the root level)
 const double* cascade_above(its_svo s, int stop, hipStream_t st) {
     const double* prev = nullptr;
-    int Dprev = 0, k = 0;
-    for (int d = s->depth - 1; d > stop; --d, k ^= 1) {
+    int Dprev = 0, k = 0, d = s->depth - 1;
+    // both buffers sized for the largest level first: the pointers taken
+    // below must not move
+    size_t most = 1;
+    for (int e = s->depth - 1; e > stop; --e) most = std::max(most, (size_t)(s->lv[e].R + 1) * (s->lv[e].R + 1) * (s->lv[e].R + 1));
+    s->casc[0].ensure(most);
+    s->casc[1].ensure(most);
+    // the small top levels in one launch
+    TopLevels T{};
+    for (; d > stop && s->lv[d].R + 1 <= kTopD; --d, k ^= 1) {
         const int D = s->lv[d].R + 1;
-        s->casc[k].ensure((size_t)D * D * D);
+        T.D[T.n] = D;
+        T.lat[T.n] = s->lv[d].lattice.p;
+        T.voff[T.n] = s->esum_verts[d];
+        T.out[T.n] = s->casc[k].p;
+        ++T.n;
+        prev = s->casc[k].p;
+        Dprev = D;
+    }
+    if (T.n) {
+        hipLaunchKernelGGL(k_dense_top, dim3(1), dim3(kTopThreads), 0, st, T, (const double*)s->lambda64.p);
+        ITS_LAUNCH_CHECK();
+    }
+    for (; d > stop; --d, k ^= 1) {
+        const int D = s->lv[d].R + 1;
         launch_dense_level<double>(s, d, prev, Dprev, s->casc[k].p, st);
         prev = s->casc[k].p;
         Dprev = D;

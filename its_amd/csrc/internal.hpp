@@ -74,10 +74,11 @@ struct its_svo_s {
                                       // child link in the 5 low bits of each
     int dense_level = 0;
     // grow-only work space: two binary64 levels of the dense cascade, and the
-    // marching-cubes lattice, soup and per-wave vertex counts
+    // marching-cubes lattice, cube cases, per-step vertex counts and soup
     its::DBuf<double> casc[2];
     its::DBuf<float> mc_lat, mc_soup;
-    its::DBuf<uint64_t> mc_wave;
+    its::DBuf<uint8_t> mc_case;
+    its::DBuf<uint64_t> mc_step;
     its::DBuf<unsigned long long> mc_valid;
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};

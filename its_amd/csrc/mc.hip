@@ -7,13 +7,12 @@
 // The reference evaluates f at 8 corners per voxel.  Each corner is a brute
 // force sum over all n basis functions, batched over 1024 streams.  Here f is
 // evaluated once per lattice point ((res+1)^3 points), by the dense cascade on
-// the octree's own lattice or by the octree field walk elsewhere.  Extraction
-// is two passes of the same walk, each wave over whole rows of voxels: a count
-// pass writes one vertex count per wave, an exclusive scan of those gives each
-// wave its output offset, and an emit pass re-derives the cube cases and
-// places each voxel's triangles with a wave prefix sum.  No per-voxel arrays
-// are written.  The soup order is the reference's: voxels x-fastest,
-// triangles in table order.
+// the octree's own lattice or by the octree field walk elsewhere.  A count
+// pass writes every voxel's cube case and the vertex count of every step of
+// 64 voxels along x; an exclusive scan of the step counts gives each step its
+// offset; an emit pass, one wave per step, places each voxel's triangles with
+// a wave prefix sum, and steps without triangles end after one load.  The
+// soup order is the reference's: voxels x-fastest, triangles in table order.
 // The 1e-6 snaps of vertexLerp (MarchingCubes.cu:72-90) are kept, and so is
 // the lerp fma(t, v1, fma(-t, v0, v0)).
 #include <algorithm>
@@ -36,50 +35,66 @@ __constant__ unsigned char c_cx[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __constant__ unsigned char c_cy[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 __constant__ unsigned char c_cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
-// The voxel grid as rows along x in the reference's voxel order (x fastest,
-// then y, then z): row r = j + ry * k.  Each wave walks rows_per_wave
-// consecutive rows, 64 voxels of a row at a time, in both passes.
+// The voxel grid in the reference's voxel order (x fastest, then y, then z),
+// cut into steps of 64 voxels along x: step s = (j + ry * k) * nch + chunk.
+// The count pass gives each wave one 64-voxel column chunk over J rows of
+// one z-slab: walking +y, the lattice rows at j+1 become those at j, so a
+// step loads two new rows (x and x+1 for the planes k and k+1), prefetched
+// one row ahead.  It writes each voxel's cube case and each step's vertex
+// count; a scan of the step counts gives each step its place in the soup.
 struct McGrid {
     int rx, ry, rz;
     int64_t lx, ly;   // lattice points per axis in x, y (res + 1)
-    int64_t nrows;    // ry * rz
-    int64_t nwaves;   // ceil(nrows / rows_per_wave)
-    int rows_per_wave;
+    int nch;          // 64-voxel chunks per row
+    int J;            // rows per count wave
+    int njb;          // ceil(ry / J)
+    int64_t nwaves;   // count waves: nch * njb * rz
+    int64_t nsteps;   // ry * rz * nch
     float iso;
 };
 
-// cube case of voxel (i, j, k); base = lattice index of point (0, j, k)
-__device__ inline unsigned mc_case(const float* lat, const McGrid& G, int64_t base, int i) {
+__device__ inline unsigned case8(const float (&v)[8], float iso) {
     unsigned c = 0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-        c |= (unsigned)(lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]] < G.iso) << q;
+    for (int q = 0; q < 8; ++q) c |= (unsigned)(v[q] < iso) << q;
     return c;
 }
 
-// pass 1: vertices per wave, and the number of voxels with triangles
-__global__ __launch_bounds__(kT) void k_mc_count(const float* lat, McGrid G, uint64_t* wave_verts,
-                                                 unsigned long long* nvalid) {
+// pass 1: cube cases, vertices per step, voxels with triangles
+__global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, McGrid G, uint8_t* __restrict__ cases,
+                                                 uint64_t* __restrict__ step_verts, unsigned long long* nvalid) {
     const int lane = threadIdx.x & 63;
     const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
     if (w >= G.nwaves) return;   // whole waves only
-    const int64_t r0 = w * G.rows_per_wave, r1 = min(r0 + G.rows_per_wave, G.nrows);
-    uint32_t nv = 0, nval = 0;
-    for (int64_t r = r0; r < r1; ++r) {
-        const int64_t j = r % G.ry, k = r / G.ry;
-        const int64_t base = (k * G.ly + j) * G.lx;
-        for (int i = lane; i < G.rx; i += 64) {
-            const unsigned n = c_nv[mc_case(lat, G, base, i)];
-            nv += n;
-            nval += n != 0;
-        }
+    const int ch = (int)(w % G.nch);
+    const int64_t rest = w / G.nch;
+    const int jb = (int)(rest % G.njb), k = (int)(rest / G.njb);
+    const int j0 = jb * G.J, j1 = min(j0 + G.J, G.ry);
+    const int i = ch * 64 + lane;
+    const bool in = i < G.rx;
+    const int ii = in ? i : G.rx - 1;   // lanes past the row read in bounds and count nothing
+    const int64_t p0 = (int64_t)k * G.ly * G.lx + ii, p1 = p0 + G.ly * G.lx;
+    // lattice rows at j (a: plane k, b: plane k + 1), each at x and x + 1
+    float a0 = lat[p0 + j0 * G.lx], a1 = lat[p0 + j0 * G.lx + 1];
+    float b0 = lat[p1 + j0 * G.lx], b1 = lat[p1 + j0 * G.lx + 1];
+    float na0 = lat[p0 + (j0 + 1) * G.lx], na1 = lat[p0 + (j0 + 1) * G.lx + 1];
+    float nb0 = lat[p1 + (j0 + 1) * G.lx], nb1 = lat[p1 + (j0 + 1) * G.lx + 1];
+    unsigned nval = 0;
+    for (int j = j0; j < j1; ++j) {
+        const int64_t jn = min(j + 2, G.ry) * G.lx;
+        const float f0 = lat[p0 + jn], f1 = lat[p0 + jn + 1], g0 = lat[p1 + jn], g1 = lat[p1 + jn + 1];
+        // the reference's corner order: (x,y,z) = (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1)
+        const float v[8] = {a0, na0, na1, a1, b0, nb0, nb1, b1};
+        const unsigned c = case8(v, G.iso);
+        const unsigned nv = in ? (unsigned)c_nv[c] : 0u;
+        if (in) cases[((int64_t)k * G.ry + j) * G.rx + i] = (uint8_t)c;
+        const unsigned sum = wave_sum(nv);
+        nval += __popcll(__ballot(nv != 0));
+        if (lane == 0) step_verts[((int64_t)k * G.ry + j) * G.nch + ch] = sum;
+        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
+        na0 = f0; na1 = f1; nb0 = g0; nb1 = g1;
     }
-    nv = wave_sum(nv);
-    nval = wave_sum(nval);
-    if (lane == 0) {
-        wave_verts[w] = nv;
-        if (nval) atomicAdd(nvalid, (unsigned long long)nval);
-    }
+    if (lane == 0 && nval) atomicAdd(nvalid, (unsigned long long)nval);
 }
 
 struct P3 {
@@ -112,56 +127,51 @@ __device__ inline P3 corner_pos(const P3& vp, double vx, double vy, double vz, i
               vp.z + (((kCornerZ >> q) & 1) ? vz : 0.0)};
 }
 
-// pass 2: the same walk; a wave prefix sum over the 64 voxels of a step places
-// their triangles after those of all earlier voxels (wave_off: exclusive scan
-// of pass 1)
-__global__ __launch_bounds__(kT) void k_mc_emit(const float* lat, McGrid G, double ox, double oy, double oz,
-                                                double vx, double vy, double vz, const uint64_t* wave_off,
-                                                uint64_t max_verts, float* out) {
+// pass 2: one wave per step; a step without vertices ends after one load.  A
+// wave prefix sum over its 64 voxels places their triangles after those of
+// all earlier voxels (step_off: exclusive scan of the step counts).
+__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
+                                                double oz, double vx, double vy, double vz,
+                                                const uint8_t* __restrict__ cases, const uint64_t* __restrict__ step_off,
+                                                uint64_t total, uint64_t max_verts, float* __restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
-    if (w >= G.nwaves) return;
-    const int64_t r0 = w * G.rows_per_wave, r1 = min(r0 + G.rows_per_wave, G.nrows);
-    uint64_t at = wave_off[w];
-    const double iso = (double)G.iso;
-    for (int64_t r = r0; r < r1; ++r) {
-        const int64_t j = r % G.ry, k = r / G.ry;
-        const int64_t base = (k * G.ly + j) * G.lx;
-        for (int i0 = 0; i0 < G.rx; i0 += 64) {
-            const int i = i0 + lane;
-            unsigned c = 0, nv = 0;
-            if (i < G.rx) {
-                c = mc_case(lat, G, base, i);
-                nv = c_nv[c];
-            }
-            if (__ballot(nv != 0) == 0) continue;
-            const uint32_t incl = wave_incl_scan(nv);
-            const uint32_t step = __shfl(incl, 63, 64);
-            if (nv) {
-                // voxelToMeshKernel: voxelPos = shift * voxelSize + origin
-                const P3 vp{(double)i * vx + ox, (double)j * vy + oy, (double)k * vz + oz};
-                double sv[8];
+    const int64_t s = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+    if (s >= G.nsteps) return;
+    const uint64_t at = step_off[s];
+    if ((s + 1 < G.nsteps ? step_off[s + 1] : total) == at) return;   // no vertices in this step
+    const int ch = (int)(s % G.nch);
+    const int64_t row = s / G.nch;
+    const int j = (int)(row % G.ry), k = (int)(row / G.ry);
+    const int i = ch * 64 + lane;
+    unsigned c = 0, nv = 0;
+    if (i < G.rx) {
+        c = cases[row * G.rx + i];
+        nv = c_nv[c];
+    }
+    const uint32_t incl = wave_incl_scan(nv);
+    if (!nv) return;
+    const int64_t base = ((int64_t)k * G.ly + j) * G.lx;
+    // voxelToMeshKernel: voxelPos = shift * voxelSize + origin
+    const P3 vp{(double)i * vx + ox, (double)j * vy + oy, (double)k * vz + oz};
+    double sv[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) sv[q] = (double)lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]];
-                const uint4 row = *reinterpret_cast<const uint4*>(&c_tri[c][0]);
-                const uint64_t first = at + incl - nv;
-                for (unsigned t = 0; t < nv; t += 3) {
-                    const uint64_t pos = first + t;
-                    if (pos >= max_verts - 3) break;
-                    for (unsigned q = 0; q < 3; ++q) {
-                        const unsigned m = t + q;
-                        const unsigned word = (m >> 2) == 0 ? row.x : (m >> 2) == 1 ? row.y : (m >> 2) == 2 ? row.z : row.w;
-                        const int e = (int)(signed char)(word >> (8 * (m & 3)));
-                        const int qa = edge_a(e), qb = edge_b(e);
-                        const P3 p = vlerp(corner_pos(vp, vx, vy, vz, qa), corner_pos(vp, vx, vy, vz, qb), pick8(sv, qa),
-                                           pick8(sv, qb), iso);
-                        out[3 * (pos + q) + 0] = (float)p.x;
-                        out[3 * (pos + q) + 1] = (float)p.y;
-                        out[3 * (pos + q) + 2] = (float)p.z;
-                    }
-                }
-            }
-            at += step;
+    for (int q = 0; q < 8; ++q) sv[q] = (double)lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]];
+    const uint4 tri = *reinterpret_cast<const uint4*>(&c_tri[c][0]);
+    const uint64_t first = at + incl - nv;
+    const double iso = (double)G.iso;
+    for (unsigned t = 0; t < nv; t += 3) {
+        const uint64_t pos = first + t;
+        if (pos >= max_verts - 3) break;
+        for (unsigned q = 0; q < 3; ++q) {
+            const unsigned m = t + q;
+            const unsigned word = (m >> 2) == 0 ? tri.x : (m >> 2) == 1 ? tri.y : (m >> 2) == 2 ? tri.z : tri.w;
+            const int e = (int)(signed char)(word >> (8 * (m & 3)));
+            const int qa = edge_a(e), qb = edge_b(e);
+            const P3 p = vlerp(corner_pos(vp, vx, vy, vz, qa), corner_pos(vp, vx, vy, vz, qb), pick8(sv, qa),
+                               pick8(sv, qb), iso);
+            out[3 * (pos + q) + 0] = (float)p.x;
+            out[3 * (pos + q) + 1] = (float)p.y;
+            out[3 * (pos + q) + 2] = (float)p.z;
         }
     }
 }
@@ -225,6 +235,7 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     auto t0 = std::chrono::steady_clock::now();
     double vs[3] = {width[0] / res[0], width[1] / res[1], width[2] / res[2]};   // MarchingCubes.cu:682
     int64_t nvox = (int64_t)res[0] * res[1] * res[2];
+    ITS_REQUIRE(nvox <= ((int64_t)1 << 33), ITS_E_RANGE, "marching-cubes grid too large");
     ITS_HIP(hipEventRecord(ev[0], st));
     lattice_values(s, origin, width, res, s->mc_lat, st);
     ITS_HIP(hipEventRecord(ev[1], st));
@@ -236,27 +247,29 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     G.rz = res[2];
     G.lx = res[0] + 1;
     G.ly = res[1] + 1;
-    G.nrows = (int64_t)res[1] * res[2];
-    // whole rows per wave, about 32 waves per CU over the grid
-    const int64_t target = 32 * (int64_t)std::max(1, s->ctx->prop.multiProcessorCount);
-    G.rows_per_wave = (int)std::max<int64_t>(1, (G.nrows + target - 1) / target);
-    G.nwaves = (G.nrows + G.rows_per_wave - 1) / G.rows_per_wave;
+    G.nch = (res[0] + 63) / 64;
+    G.J = 32;
+    G.njb = (res[1] + G.J - 1) / G.J;
+    G.nwaves = (int64_t)G.nch * G.njb * res[2];
+    G.nsteps = (int64_t)res[1] * res[2] * G.nch;
     G.iso = (float)iso;
-    const unsigned nb = (unsigned)((G.nwaves * 64 + kT - 1) / kT);
-    s->mc_wave.ensure(G.nwaves);
+    s->mc_case.ensure(nvox);
+    s->mc_step.ensure(G.nsteps);
     s->mc_valid.ensure(1);
     ITS_HIP(hipMemsetAsync(s->mc_valid.p, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_mc_count, dim3(nb), dim3(kT), 0, st, (const float*)s->mc_lat.p, G, s->mc_wave.p,
-                       s->mc_valid.p);
+    hipLaunchKernelGGL(k_mc_count, dim3((unsigned)((G.nwaves * 64 + kT - 1) / kT)), dim3(kT), 0, st,
+                       (const float*)s->mc_lat.p, G, s->mc_case.p, s->mc_step.p, s->mc_valid.p);
     ITS_LAUNCH_CHECK();
-    const uint64_t total = scan_exclusive_u64(s->mc_wave.p, s->mc_wave.p, G.nwaves, st);
+    const uint64_t total = scan_exclusive_u64(s->mc_step.p, s->mc_step.p, G.nsteps, st);
     S.num_verts = (int64_t)total;
     const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
     if (h_tri && total) {
         ITS_REQUIRE((int64_t)total <= capacity, ITS_E_RANGE, "triangle buffer too small");
         s->mc_soup.ensure(3 * total);
-        hipLaunchKernelGGL(k_mc_emit, dim3(nb), dim3(kT), 0, st, (const float*)s->mc_lat.p, G, origin[0], origin[1],
-                           origin[2], vs[0], vs[1], vs[2], (const uint64_t*)s->mc_wave.p, max_verts, s->mc_soup.p);
+        hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((G.nsteps * 64 + kT - 1) / kT)), dim3(kT), 0, st,
+                           (const float*)s->mc_lat.p, G, origin[0], origin[1], origin[2], vs[0], vs[1], vs[2],
+                           (const uint8_t*)s->mc_case.p, (const uint64_t*)s->mc_step.p, total, max_verts,
+                           s->mc_soup.p);
         ITS_LAUNCH_CHECK();
         ITS_HIP(hipEventRecord(ev[2], st));
         ITS_HIP(hipMemcpyAsync(h_tri, s->mc_soup.p, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
