@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -43,6 +44,9 @@ namespace its {
 namespace {
 constexpr int kT = 256;
 constexpr int kBucket = 8;
+// query walks treat nodes of this level (at most 64 triangles) as leaves
+constexpr int kLeafLevel = 3;
+static_assert((kBucket << kLeafLevel) == 64, "one triangle per lane in a leaf");
 constexpr int kMaxLevels = 32;
 
 // ------------------------------------------------------------ geometry ----
@@ -131,6 +135,25 @@ __device__ inline float solid_angle(const double p[3], const double* A, const do
     return 2.0f * atan2_fit(det, den);
 }
 
+// apex of a node's fan: the centre of its binary32 box, in binary64
+__host__ __device__ inline double fan_apex(float lo, float hi) { return 0.5 * ((double)lo + (double)hi); }
+
+// solid angle of the fan triangle (apex, a, b) seen from p, from the
+// binary32 vectors z = apex - p (|z| = lz), u = a - p, v = b - p: the same
+// formula as solid_angle with its first vertex fixed
+__device__ inline float fan_angle(const float z[3], float lz, const float u[3], const float v[3]) {
+#pragma clang fp contract(fast)
+    const float lu = __builtin_amdgcn_sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const float lv = __builtin_amdgcn_sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const float det = z[0] * (u[1] * v[2] - u[2] * v[1]) + z[1] * (u[2] * v[0] - u[0] * v[2]) +
+                      z[2] * (u[0] * v[1] - u[1] * v[0]);
+    const float zu = z[0] * u[0] + z[1] * u[1] + z[2] * u[2];
+    const float uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    const float vz = v[0] * z[0] + v[1] * z[1] + v[2] * z[2];
+    const float den = lz * lu * lv + zu * lv + uv * lz + vz * lu;
+    return 2.0f * atan2_fit(det, den);
+}
+
 // binary32 neighbours of a binary64 value, rounded down / up
 __device__ inline float f_down(double x) {
     float f = (float)x;
@@ -212,7 +235,9 @@ struct TreeView {
     const uint32_t* code;    // centroid Morton code per sorted triangle (ascending)
     const float4* node;      // 2 per node: box lo.xyz + first fan edge (int bits in w),
                              // box hi.xyz + fan edge count (int bits in w; -1: no fan)
-    const double* fan;       // 6 per fan edge: a xyz, b xyz
+    const float* fan;        // 6 per fan edge: a - centre, b - centre (centre of its node's box)
+    const uint32_t* cell_lb; // per cell_bits-bit code prefix C: first sorted triangle with code >= C's range
+    int cell_bits;
 };
 
 // -------------------------------------------------------------- build -----
@@ -362,30 +387,30 @@ __global__ void k_fan_node_count(const uint64_t* keys, const uint32_t* vals, int
     });
 }
 
-__device__ inline bool fan_kept(const Shape& s, const uint32_t* node_cnt, int64_t node, int h, int64_t k) {
-    return (int64_t)node_cnt[node] <= node_last(s, h, k) - node_first(h, k);
-}
+// keep[node] != 0: the winding walk uses the node's fan for points outside
+// its box (chosen by fan_choice on the host)
+__device__ inline bool fan_kept(const uint8_t* keep, int64_t node) { return keep[node] != 0; }
 
 __global__ void k_fan_edge_count(const uint64_t* keys, const uint32_t* vals, int64_t M, Shape s, uint64_t none,
-                                 const uint32_t* node_cnt, uint32_t* cnt) {
+                                 const uint8_t* keep, uint32_t* cnt) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     uint32_t c = 0;
     if (edge_head(keys, i, none))
-        edge_walk(keys, vals, i, M, s, [&](int64_t node, int h, int64_t k, int coef) {
-            if (fan_kept(s, node_cnt, node, h, k)) c += (uint32_t)abs(coef);
+        edge_walk(keys, vals, i, M, s, [&](int64_t node, int, int64_t, int coef) {
+            if (fan_kept(keep, node)) c += (uint32_t)abs(coef);
         });
     cnt[i] = c;
 }
 
 __global__ void k_fan_emit(const uint64_t* keys, const uint32_t* vals, int64_t M, Shape s, uint64_t none, int vb,
-                           const uint32_t* node_cnt, const uint32_t* off, uint32_t* fan_node, uint64_t* fan_edge) {
+                           const uint8_t* keep, const uint32_t* off, uint32_t* fan_node, uint64_t* fan_edge) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M || !edge_head(keys, i, none)) return;
     const uint64_t lo = keys[i] >> vb, hi = keys[i] & ((1ull << vb) - 1);
     uint32_t o = off[i];
-    edge_walk(keys, vals, i, M, s, [&](int64_t node, int h, int64_t k, int coef) {
-        if (!fan_kept(s, node_cnt, node, h, k)) return;
+    edge_walk(keys, vals, i, M, s, [&](int64_t node, int, int64_t, int coef) {
+        if (!fan_kept(keep, node)) return;
         const uint64_t e = coef > 0 ? ((lo << 32) | hi) : ((hi << 32) | lo);
         for (int r = 0; r < abs(coef); ++r) {
             fan_node[o] = (uint32_t)node;
@@ -395,36 +420,62 @@ __global__ void k_fan_emit(const uint64_t* keys, const uint32_t* vals, int64_t M
     });
 }
 
-__global__ void k_fan_table(Shape s, int64_t nodes, const uint32_t* node_cnt, const uint32_t* fan_off, float4* node) {
+__global__ void k_fan_table(int64_t nodes, const uint32_t* node_cnt, const uint8_t* keep, const uint32_t* fan_off,
+                            float4* node) {
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nodes) return;
-    int h = 0;
-    while (h < s.H && n >= s.off[h + 1]) ++h;
-    const int64_t k = n - s.off[h];
-    const bool kept = h > 0 && fan_kept(s, node_cnt, n, h, k);
+    const bool kept = fan_kept(keep, n);
     node[2 * n].w = __int_as_float((int)fan_off[n]);
     node[2 * n + 1].w = __int_as_float(kept ? (int)node_cnt[n] : -1);
 }
 
-__global__ void k_fan_coords(const uint64_t* fan_edge, int64_t E, const double* V, double* fan) {
+// fan edge ends relative to the centre of their node's box (the fan apex),
+// rounded once to binary32
+__global__ void k_fan_coords(const uint64_t* fan_edge, const uint32_t* fan_node, int64_t E, const double* V,
+                             const float4* node, float* fan) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     const uint64_t ab = fan_edge[e];
     const int64_t a = (int64_t)(ab >> 32), b = (int64_t)(ab & 0xffffffffull);
+    const int64_t n = fan_node[e];
+    const double c[3] = {fan_apex(node[2 * n].x, node[2 * n + 1].x), fan_apex(node[2 * n].y, node[2 * n + 1].y),
+                         fan_apex(node[2 * n].z, node[2 * n + 1].z)};
     for (int x = 0; x < 3; ++x) {
-        fan[6 * e + x] = V[3 * a + x];
-        fan[6 * e + 3 + x] = V[3 * b + x];
+        fan[6 * e + x] = (float)(V[3 * a + x] - c[x]);
+        fan[6 * e + 3 + x] = (float)(V[3 * b + x] - c[x]);
     }
 }
 
 // kept fan edges per node (0 for dropped fans and buckets), for the scan
-__global__ void k_fan_kept_count(Shape s, int64_t nodes, const uint32_t* node_cnt, uint32_t* kept) {
+__global__ void k_fan_kept_count(int64_t nodes, const uint32_t* node_cnt, const uint8_t* keep, uint32_t* kept) {
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nodes) return;
-    int h = 0;
-    while (h < s.H && n >= s.off[h + 1]) ++h;
-    const int64_t k = n - s.off[h];
-    kept[n] = (h > 0 && fan_kept(s, node_cnt, n, h, k)) ? node_cnt[n] : 0u;
+    kept[n] = fan_kept(keep, n) ? node_cnt[n] : 0u;
+}
+
+// Which nodes keep their fan, bottom-up with a cost per point outside the
+// node: a visit is one node read; a fan adds ceil(edges / 64) rounds of
+// solid angles, a leaf one round of its triangles, and descending the costs
+// of the children.  A node keeps its fan when that is no dearer than
+// descending.  Either way the winding number is exact; only the work varies.
+// Nodes below the leaf level are never visited and keep nothing.
+std::vector<uint8_t> fan_choice(const Shape& s, int leaf, const std::vector<uint32_t>& node_cnt) {
+    std::vector<uint8_t> keep(node_cnt.size(), 0);
+    std::vector<double> cost(node_cnt.size(), 0.0);
+    for (int h = leaf; h <= s.H; ++h)
+        for (int64_t k = 0; k < s.count[h]; ++k) {
+            const int64_t n = s.off[h] + k;
+            if (h == leaf) {
+                cost[n] = 2.0;
+                continue;
+            }
+            const int64_t c0 = s.off[h - 1] + 2 * k;
+            const double down = 1.0 + cost[c0] + (2 * k + 1 < s.count[h - 1] ? cost[c0 + 1] : 0.0);
+            const double fan = 1.0 + std::ceil(node_cnt[n] / 64.0);
+            keep[n] = fan <= down;
+            cost[n] = std::min(fan, down);
+        }
+    return keep;
 }
 
 int bits_for(uint64_t maxval) {
@@ -458,14 +509,30 @@ struct TriTree {
     Shape s{};
     Quant q{};
     int64_t nodes = 0, fans = 0;
-    DBuf<double> tris, fan;
+    DBuf<double> tris;
+    DBuf<float> fan;
     DBuf<float4> sph, node;
     DBuf<int32_t> tid;
-    DBuf<uint32_t> code;
+    DBuf<uint32_t> code, cell_lb;
+    int cell_bits = 0;
     TreeView view() const {
-        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p};
+        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p, cell_lb.p, cell_bits};
     }
 };
+
+// lower bound of each code prefix's range in the sorted codes: entry C
+// (0 <= C <= 2^bits) is the first position whose code is >= C << (30 - bits)
+__global__ void k_cell_lb(const uint32_t* code, int64_t T, int bits, uint32_t* lb) {
+    const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (C > ((int64_t)1 << bits)) return;
+    const uint64_t target = (uint64_t)C << (30 - bits);
+    int64_t a = 0, b = T;
+    while (a < b) {
+        const int64_t m = (a + b) >> 1;
+        if ((uint64_t)code[m] < target) a = m + 1; else b = m;
+    }
+    lb[C] = (uint32_t)a;
+}
 
 void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, const int32_t* d_F, int64_t nF) {
     ITS_REQUIRE(nF > 0 && nF < (1ll << 30) && nV < (1ll << 31), ITS_E_RANGE, "mesh too large for the triangle tree (2^30 triangles, 2^31 vertices)");
@@ -511,6 +578,14 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     hipLaunchKernelGGL(k_tri_gather, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, tkeys.p, tb, nF, d_V, d_F, tr.tris.p,
                        tr.sph.p, tr.tid.p, tr.code.p, pos.p);
     ITS_LAUNCH_CHECK();
+    // code-prefix table with about one prefix per triangle (3 to 21 bits)
+    tr.cell_bits = 3;
+    while (tr.cell_bits < 21 && ((int64_t)1 << tr.cell_bits) < nF) ++tr.cell_bits;
+    const int64_t ncell = ((int64_t)1 << tr.cell_bits) + 1;
+    tr.cell_lb.alloc(ncell);
+    hipLaunchKernelGGL(k_cell_lb, dim3(blocks_for(ncell, kT)), dim3(kT), 0, st, (const uint32_t*)tr.code.p, nF,
+                       tr.cell_bits, tr.cell_lb.p);
+    ITS_LAUNCH_CHECK();
 
     // node boxes, one level per launch
     tr.node.alloc(2 * tr.nodes);
@@ -537,9 +612,18 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     hipLaunchKernelGGL(k_fan_node_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
                        node_cnt.p);
     ITS_LAUNCH_CHECK();
+    DBuf<uint8_t> keep(tr.nodes);
+    {
+        std::vector<uint32_t> h_cnt(tr.nodes);
+        ITS_HIP(hipMemcpyAsync(h_cnt.data(), node_cnt.p, tr.nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+        const std::vector<uint8_t> h_keep = fan_choice(s, std::min(kLeafLevel, (int)s.H), h_cnt);
+        ITS_HIP(hipMemcpyAsync(keep.p, h_keep.data(), tr.nodes, hipMemcpyHostToDevice, st));
+        ITS_HIP(hipStreamSynchronize(st));
+    }
     DBuf<uint32_t> ecnt(M);
     hipLaunchKernelGGL(k_fan_edge_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
-                       (const uint32_t*)node_cnt.p, ecnt.p);
+                       (const uint8_t*)keep.p, ecnt.p);
     ITS_LAUNCH_CHECK();
     const uint64_t E = scan_exclusive_u32(ecnt.p, ecnt.p, M, st);
     ITS_REQUIRE(E < (1ull << 31), ITS_E_RANGE, "triangle tree: more than 2^31 boundary fan edges");
@@ -548,24 +632,24 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     DBuf<uint64_t> fan_edge(std::max<uint64_t>(E, 1));
     if (E) {
         hipLaunchKernelGGL(k_fan_emit, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none, vb,
-                           (const uint32_t*)node_cnt.p, (const uint32_t*)ecnt.p, fan_node.p, fan_edge.p);
+                           (const uint8_t*)keep.p, (const uint32_t*)ecnt.p, fan_node.p, fan_edge.p);
         ITS_LAUNCH_CHECK();
         // stable: within a node, fan edges stay in edge-key order
         radix_sort_pairs(fan_node, fan_edge, (int64_t)E, bits_for((uint64_t)tr.nodes), st);
     }
     DBuf<uint32_t> fan_off(tr.nodes);
-    hipLaunchKernelGGL(k_fan_kept_count, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, s, tr.nodes,
-                       (const uint32_t*)node_cnt.p, fan_off.p);
+    hipLaunchKernelGGL(k_fan_kept_count, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, tr.nodes,
+                       (const uint32_t*)node_cnt.p, (const uint8_t*)keep.p, fan_off.p);
     ITS_LAUNCH_CHECK();
     const uint64_t E2 = scan_exclusive_u32(fan_off.p, fan_off.p, tr.nodes, st);
     ITS_REQUIRE(E2 == E, ITS_E_STATE, "triangle tree: fan edge counts disagree");
-    hipLaunchKernelGGL(k_fan_table, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, s, tr.nodes,
-                       (const uint32_t*)node_cnt.p, (const uint32_t*)fan_off.p, tr.node.p);
+    hipLaunchKernelGGL(k_fan_table, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, tr.nodes,
+                       (const uint32_t*)node_cnt.p, (const uint8_t*)keep.p, (const uint32_t*)fan_off.p, tr.node.p);
     ITS_LAUNCH_CHECK();
     tr.fan.alloc(6 * std::max<uint64_t>(E, 1));
     if (E) {
-        hipLaunchKernelGGL(k_fan_coords, dim3(blocks_for((int64_t)E, kT)), dim3(kT), 0, st, fan_edge.p, (int64_t)E,
-                           d_V, tr.fan.p);
+        hipLaunchKernelGGL(k_fan_coords, dim3(blocks_for((int64_t)E, kT)), dim3(kT), 0, st, (const uint64_t*)fan_edge.p,
+                           (const uint32_t*)fan_node.p, (int64_t)E, d_V, (const float4*)tr.node.p, tr.fan.p);
         ITS_LAUNCH_CHECK();
     }
     ITS_HIP(hipStreamSynchronize(st));
@@ -576,8 +660,6 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
 // lanes, and nodes of level kLeafLevel and below (at most 64 triangles) are
 // leaves whose triangles the 64 lanes take one each.  Per-lane results are
 // combined by a fixed butterfly, so nothing depends on timing.
-constexpr int kLeafLevel = 3;
-static_assert((kBucket << kLeafLevel) == 64, "one triangle per lane in a leaf");
 constexpr int kWavesPerBlock = kT / 64;
 
 struct NodeRec {
@@ -667,8 +749,10 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
         const float pm = fmaxf(fabsf(pf[0]), fmaxf(fabsf(pf[1]), fabsf(pf[2])));
         double best = INFINITY;
         int32_t bt = 0x7fffffff;
+        // the lower bound of c among the sorted codes lies within its prefix's range
         const uint32_t c = quant_code(t.q, p);
-        int64_t a = 0, b = t.s.T;
+        const uint32_t C = c >> (30 - t.cell_bits);
+        int64_t a = t.cell_lb[C], b = t.cell_lb[C + 1];
         while (a < b) {
             const int64_t m = (a + b) >> 1;
             if (t.code[m] < c) a = m + 1; else b = m;
@@ -714,11 +798,16 @@ __global__ __launch_bounds__(kT) void k_sdf_wind(TreeView t, const double* pts, 
                     acc += solid_angle(p, T, T + 3, T + 6);
                 }
             } else if (r.count >= 0 && box_outside(p, r.lo, r.hi)) {
-                const double c[3] = {0.5 * ((double)r.lo.x + (double)r.hi.x), 0.5 * ((double)r.lo.y + (double)r.hi.y),
-                                     0.5 * ((double)r.lo.z + (double)r.hi.z)};
+                // apex - p once per node in binary64; the edge ends are
+                // stored relative to the apex, so a - p = (a - apex) + (apex - p)
+                const float z[3] = {(float)(fan_apex(r.lo.x, r.hi.x) - p[0]), (float)(fan_apex(r.lo.y, r.hi.y) - p[1]),
+                                    (float)(fan_apex(r.lo.z, r.hi.z) - p[2])};
+                const float lz = __builtin_amdgcn_sqrtf(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
                 for (int e = lane; e < r.count; e += 64) {
-                    const double* ab = t.fan + 6 * ((int64_t)r.first + e);
-                    acc += solid_angle(p, c, ab, ab + 3);
+                    const float* ab = t.fan + 6 * ((int64_t)r.first + e);
+                    const float u[3] = {ab[0] + z[0], ab[1] + z[1], ab[2] + z[2]};
+                    const float v[3] = {ab[3] + z[0], ab[4] + z[1], ab[5] + z[2]};
+                    acc += fan_angle(z, lz, u, v);
                 }
             } else {
                 --h;
