@@ -718,7 +718,7 @@ __device__ inline void leaf_nearest(const TreeView& t, int h, int64_t k, int lan
 // true when no point of the box can be within the best distance found so
 // far: binary32 distance to the box (boxes are rounded outward, so they hold
 // every vertex), reduced by a margin that covers its rounding
-__device__ inline bool box_beyond(const float pf[3], float pm, float4 lo, float4 hi, double best) {
+__device__ inline float box_lb(const float pf[3], float pm, float4 lo, float4 hi) {
     const float l[3] = {lo.x, lo.y, lo.z}, u[3] = {hi.x, hi.y, hi.z};
     float d2 = 0.f, sc = pm;
     for (int a = 0; a < 3; ++a) {
@@ -726,8 +726,18 @@ __device__ inline bool box_beyond(const float pf[3], float pm, float4 lo, float4
         d2 += e * e;
         sc = fmaxf(sc, fmaxf(fabsf(l[a]), fabsf(u[a])));
     }
-    const float lb = sqrtf(d2) - bound_margin(sc);
-    return lb > 0.f && (double)lb * (double)lb > best;
+    return sqrtf(d2) - bound_margin(sc);
+}
+__device__ inline bool lb_beyond(float lb, double best) { return lb > 0.f && (double)lb * (double)lb > best; }
+__device__ inline bool box_beyond(const float pf[3], float pm, float4 lo, float4 hi, double best) {
+    return lb_beyond(box_lb(pf, pm, lo, hi), best);
+}
+
+// the writes of some lanes are read by other lanes of the same wavefront
+__device__ inline void lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
@@ -738,39 +748,117 @@ __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
 // Nearest triangle: binary64 distance (unsigned, into out) and the original
 // index of the nearest triangle.  The walk starts with the leaf at the
 // point's Morton position, which usually holds the nearest triangle.
+// Distance walk, level by level: the wavefront keeps the nodes of one level
+// that can still hold the nearest triangle (the frontier, in LDS) and tests
+// all their children at once, one child pair per lane, against the distance
+// from the seed leaf.  The surviving leaves are then searched one by one,
+// skipped when their box is beyond the best distance found so far.  A leaf
+// that can hold the nearest triangle has every ancestor box within the final
+// distance, which is no larger than the seed's, so it is never dropped: the
+// result is the exact (d², index) minimum.  A frontier that would overflow
+// LDS sends the point to the node-by-node walk.
+constexpr int kFront = 256;
+__device__ void dist_dfs(const TreeView& t, int leaf, int64_t seed, int lane, const double p[3], const float pf[3],
+                         float pm, double& best, int32_t& bt) {
+    int h = t.s.H, k = 0;
+    while (true) {
+        const NodeRec r = load_node(t, t.s.off[h] + k);
+        if (!(h == leaf && k == seed) && !box_beyond(pf, pm, r.lo, r.hi, best)) {
+            if (h > leaf) {
+                --h;
+                k <<= 1;
+                continue;
+            }
+            leaf_nearest(t, h, k, lane, p, pm, best, bt);
+        }
+        if (!tree_next(t.s, h, k)) break;
+    }
+}
+
 __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, int64_t N, double* out,
                                                  int32_t* tri_out) {
-    const int lane = threadIdx.x & 63;
+    __shared__ uint32_t fk[2][kWavesPerBlock][kFront];   // frontier node indices within their level
+    __shared__ float flb[kWavesPerBlock][kFront];        // box lower bounds of the leaf-level frontier
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
-    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); i < N; i += nw) {
+    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
         const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
         const float pm = fmaxf(fabsf(pf[0]), fmaxf(fabsf(pf[1]), fabsf(pf[2])));
         double best = INFINITY;
         int32_t bt = 0x7fffffff;
-        // the lower bound of c among the sorted codes lies within its prefix's range
         const uint32_t c = quant_code(t.q, p);
-        const uint32_t C = c >> (30 - t.cell_bits);
-        int64_t a = t.cell_lb[C], b = t.cell_lb[C + 1];
+        int64_t a = 0, b = t.s.T;
         while (a < b) {
             const int64_t m = (a + b) >> 1;
             if (t.code[m] < c) a = m + 1; else b = m;
         }
         const int64_t seed = std::min<int64_t>(a, t.s.T - 1) / ((int64_t)kBucket << leaf);
         leaf_nearest(t, leaf, seed, lane, p, pm, best, bt);
-        int h = t.s.H, k = 0;
-        while (true) {
-            const NodeRec r = load_node(t, t.s.off[h] + k);
-            if (!(h == leaf && k == seed) && !box_beyond(pf, pm, r.lo, r.hi, best)) {
-                if (h > leaf) {
-                    --h;
-                    k <<= 1;
-                    continue;
+        // frontier: the root, unless its box is beyond the seed's distance
+        const NodeRec root = load_node(t, t.s.off[t.s.H]);
+        const float root_lb = box_lb(pf, pm, root.lo, root.hi);
+        int n = lb_beyond(root_lb, best) ? 0 : 1, cur = 0;
+        lds_fence();
+        if (lane == 0) {
+            fk[0][w][0] = 0u;
+            flb[w][0] = root_lb;
+        }
+        lds_fence();
+        bool overflow = false;
+        for (int h = t.s.H; h > leaf && n > 0; --h) {
+            const int64_t coff = t.s.off[h - 1];
+            const int ccount = t.s.count[h - 1];
+            const bool to_leaf = (h - 1 == leaf);
+            int m = 0;   // next frontier size (uniform)
+            for (int j0 = 0; j0 < n; j0 += 64) {
+                const int j = j0 + lane;
+                bool keep0 = false, keep1 = false;
+                uint32_t c0 = 0;
+                float lb0 = 0.f, lb1 = 0.f;
+                if (j < n) {
+                    c0 = 2u * fk[cur][w][j];
+                    const NodeRec r0 = load_node(t, (int)(coff + c0));
+                    lb0 = box_lb(pf, pm, r0.lo, r0.hi);
+                    keep0 = !lb_beyond(lb0, best);
+                    if ((int)c0 + 1 < ccount) {
+                        const NodeRec r1 = load_node(t, (int)(coff + c0 + 1));
+                        lb1 = box_lb(pf, pm, r1.lo, r1.hi);
+                        keep1 = !lb_beyond(lb1, best);
+                    }
                 }
-                leaf_nearest(t, h, k, lane, p, pm, best, bt);
+                const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+                const int tot = __popcll(m0) + __popcll(m1);
+                if (m + tot > kFront) {
+                    overflow = true;
+                    break;
+                }
+                const int pos = m + __popcll(m0 & below) + __popcll(m1 & below);
+                if (keep0) {
+                    fk[cur ^ 1][w][pos] = c0;
+                    if (to_leaf) flb[w][pos] = lb0;
+                }
+                if (keep1) {
+                    fk[cur ^ 1][w][pos + (keep0 ? 1 : 0)] = c0 + 1u;
+                    if (to_leaf) flb[w][pos + (keep0 ? 1 : 0)] = lb1;
+                }
+                m += tot;
             }
-            if (!tree_next(t.s, h, k)) break;
+            lds_fence();
+            if (overflow) break;
+            cur ^= 1;
+            n = m;
+        }
+        if (overflow) {
+            dist_dfs(t, leaf, seed, lane, p, pf, pm, best, bt);
+        } else {
+            for (int j = 0; j < n; ++j) {
+                const int64_t k = fk[cur][w][j];
+                if (k == seed || lb_beyond(flb[w][j], best)) continue;
+                leaf_nearest(t, leaf, k, lane, p, pm, best, bt);
+            }
         }
         if (lane == 0) {
             out[i] = sqrt(best);
@@ -780,46 +868,142 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
 }
 
 // Winding number: fans of the nodes the point is outside of, triangles of
-// the leaves it reaches.  Applies the sign to the distance in out.
+// the leaves it reaches.  Level by level: the wavefront classifies all nodes
+// of a level at once (one per lane) into leaves, fans (the point is outside
+// the box) and nodes to descend, keeping the lists in LDS; fans carry their
+// edge range and apex - p, so their sums start without another node read.
+// Lists that would overflow send the point to the node-by-node walk.
+constexpr int kWindFront = 128, kWindFans = 128, kWindLeaves = 64;
+struct FanItem {
+    int first, count;
+    float z[3], lz;
+};
+
+__device__ inline float leaf_wind(const TreeView& t, int h, int64_t k, int lane, const double p[3]) {
+    const int64_t j = node_first(h, k) + lane;
+    if (j >= node_last(t.s, h, k)) return 0.f;
+    const double* T = t.tris + 9 * j;
+    return solid_angle(p, T, T + 3, T + 6);
+}
+__device__ inline float fan_wind(const TreeView& t, int first, int count, const float z[3], float lz, int lane) {
+    float acc = 0.f;
+    for (int e = lane; e < count; e += 64) {
+        const float* ab = t.fan + 6 * ((int64_t)first + e);
+        const float u[3] = {ab[0] + z[0], ab[1] + z[1], ab[2] + z[2]};
+        const float v[3] = {ab[3] + z[0], ab[4] + z[1], ab[5] + z[2]};
+        acc += fan_angle(z, lz, u, v);
+    }
+    return acc;
+}
+__device__ inline FanItem fan_item(const NodeRec& r, const double p[3]) {
+    // apex - p once per node in binary64; the edge ends are stored relative
+    // to the apex, so a - p = (a - apex) + (apex - p)
+    FanItem f;
+    f.first = r.first;
+    f.count = r.count;
+    f.z[0] = (float)(fan_apex(r.lo.x, r.hi.x) - p[0]);
+    f.z[1] = (float)(fan_apex(r.lo.y, r.hi.y) - p[1]);
+    f.z[2] = (float)(fan_apex(r.lo.z, r.hi.z) - p[2]);
+    f.lz = __builtin_amdgcn_sqrtf(f.z[0] * f.z[0] + f.z[1] * f.z[1] + f.z[2] * f.z[2]);
+    return f;
+}
+
+// the node-by-node walk (lists overflowed)
+__device__ float wind_dfs(const TreeView& t, int leaf, int lane, const double p[3]) {
+    float acc = 0.f;
+    int h = t.s.H, k = 0;
+    while (true) {
+        const NodeRec r = load_node(t, t.s.off[h] + k);
+        if (h <= leaf) {
+            acc += leaf_wind(t, h, k, lane, p);
+        } else if (r.count >= 0 && box_outside(p, r.lo, r.hi)) {
+            const FanItem f = fan_item(r, p);
+            acc += fan_wind(t, f.first, f.count, f.z, f.lz, lane);
+        } else {
+            --h;
+            k <<= 1;
+            continue;
+        }
+        if (!tree_next(t.s, h, k)) break;
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(kT) void k_sdf_wind(TreeView t, const double* pts, int64_t N, double* out) {
-    const int lane = threadIdx.x & 63;
+    __shared__ uint32_t fk[2][kWavesPerBlock][kWindFront];   // nodes of the level being classified
+    __shared__ FanItem fans[kWavesPerBlock][kWindFans];
+    __shared__ uint32_t leaves[kWavesPerBlock][kWindLeaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
-    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); i < N; i += nw) {
+    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
-        float acc = 0.f;
-        int h = t.s.H, k = 0;
-        while (true) {
-            const NodeRec r = load_node(t, t.s.off[h] + k);
-            if (h <= leaf) {
-                const int64_t j = node_first(h, k) + lane;
-                if (j < node_last(t.s, h, k)) {
-                    const double* T = t.tris + 9 * j;
-                    acc += solid_angle(p, T, T + 3, T + 6);
+        int n = 1, cur = 0, nf = 0, nl = 0;
+        bool overflow = false;
+        lds_fence();
+        if (lane == 0) fk[0][w][0] = 0u;
+        lds_fence();
+        for (int h = t.s.H; n > 0; --h) {
+            if (h <= leaf) {   // every node of this level is a leaf
+                if (nl + n > kWindLeaves) {
+                    overflow = true;
+                    break;
                 }
-            } else if (r.count >= 0 && box_outside(p, r.lo, r.hi)) {
-                // apex - p once per node in binary64; the edge ends are
-                // stored relative to the apex, so a - p = (a - apex) + (apex - p)
-                const float z[3] = {(float)(fan_apex(r.lo.x, r.hi.x) - p[0]), (float)(fan_apex(r.lo.y, r.hi.y) - p[1]),
-                                    (float)(fan_apex(r.lo.z, r.hi.z) - p[2])};
-                const float lz = __builtin_amdgcn_sqrtf(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
-                for (int e = lane; e < r.count; e += 64) {
-                    const float* ab = t.fan + 6 * ((int64_t)r.first + e);
-                    const float u[3] = {ab[0] + z[0], ab[1] + z[1], ab[2] + z[2]};
-                    const float v[3] = {ab[3] + z[0], ab[4] + z[1], ab[5] + z[2]};
-                    acc += fan_angle(z, lz, u, v);
-                }
-            } else {
-                --h;
-                k <<= 1;
-                continue;
+                for (int j = lane; j < n; j += 64) leaves[w][nl + j] = fk[cur][w][j];
+                nl += n;
+                lds_fence();
+                break;
             }
-            if (!tree_next(t.s, h, k)) break;
+            const int64_t off = t.s.off[h];
+            const int cnext = t.s.count[h - 1];
+            int m = 0;
+            for (int j0 = 0; j0 < n; j0 += 64) {
+                const int j = j0 + lane;
+                bool is_fan = false, down = false;
+                uint32_t k = 0;
+                NodeRec r{};
+                if (j < n) {
+                    k = fk[cur][w][j];
+                    r = load_node(t, (int)(off + k));
+                    is_fan = r.count >= 0 && box_outside(p, r.lo, r.hi);
+                    down = !is_fan;
+                }
+                const bool two = down && (int)(2 * k + 1) < cnext;
+                const uint64_t mf = __ballot(is_fan), md = __ballot(down), m2 = __ballot(two);
+                const int tf = __popcll(mf), td = __popcll(md) + __popcll(m2);
+                if (nf + tf > kWindFans || m + td > kWindFront) {
+                    overflow = true;
+                    break;
+                }
+                if (is_fan) fans[w][nf + __popcll(mf & below)] = fan_item(r, p);
+                if (down) {
+                    const int pos = m + __popcll(md & below) + __popcll(m2 & below);
+                    fk[cur ^ 1][w][pos] = 2 * k;
+                    if (two) fk[cur ^ 1][w][pos + 1] = 2 * k + 1;
+                }
+                nf += tf;
+                m += td;
+            }
+            lds_fence();
+            if (overflow) break;
+            cur ^= 1;
+            n = m;
         }
-        const float w = wave_sum(acc) * (float)(0.25 / M_PI);
+        float acc = 0.f;
+        if (overflow) {
+            acc = wind_dfs(t, leaf, lane, p);
+        } else {
+            for (int f = 0; f < nf; ++f) {
+                const FanItem it = fans[w][f];
+                acc += fan_wind(t, it.first, it.count, it.z, it.lz, lane);
+            }
+            for (int l = 0; l < nl; ++l) acc += leaf_wind(t, leaf, leaves[w][l], lane, p);
+        }
+        const float wn = wave_sum(acc) * (float)(0.25 / M_PI);
         if (lane == 0) {
             const double d = out[i];
-            out[i] = (w > 0.5f) ? -d : d;
+            out[i] = (wn > 0.5f) ? -d : d;
         }
     }
 }
