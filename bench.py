@@ -137,7 +137,12 @@ def main():
     log(f"[bench] lscg cold: {t_lscg*1e3:.2f} ms, iters={st['iterations']} normal_rel={st['normal_rel']:.2e}")
 
     # marching cubes on the 256^3 grid of the octree box
-    mc_runs = [svo.marching_cubes(origin, width, (R, R, R), 0.0)[1] for _ in range(3)]
+    # the coefficients are set again before each run, so each run evaluates
+    # its lattice (the octree keeps the last one for the same coefficients)
+    mc_runs = []
+    for _ in range(3):
+        svo.set_lambda(lam)
+        mc_runs.append(svo.marching_cubes(origin, width, (R, R, R), 0.0)[1])
     mc = {"res": R, "vertices": mc_runs[-1]["num_verts"],
           "seconds_lattice": float(np.median([m["seconds_eval"] for m in mc_runs])),
           "seconds_extract": float(np.median([m["seconds_extract"] for m in mc_runs])),

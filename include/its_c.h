@@ -182,11 +182,15 @@ int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, 
  * Grid of res[0] x res[1] x res[2] voxels over origin + [0, width).  The
  * triangle soup is returned as float xyz, 3 vertices per triangle, in the
  * reference order (voxels x-fastest, triangles in table order).
- * tri_xyz may be NULL to query the count; capacity is in vertices. */
+ * tri_xyz may be NULL to query the count; capacity is in vertices.  The
+ * octree keeps the last lattice, so a fill call after its count call, or an
+ * extraction at another iso value on the same grid, evaluates no lattice. */
 typedef struct {
     int64_t num_verts;        /* allTriVertices */
     int64_t num_valid_voxels; /* nValidVoxels */
-    double seconds_eval;      /* lattice evaluation (host clock) */
+    double seconds_eval;      /* lattice evaluation (host clock); near 0 when the
+                                 lattice of the previous call on this octree was
+                                 reused (same grid, coefficients unchanged) */
     double seconds_extract;   /* count, scan and emission, with the copy of the
                                  soup to the caller's buffer (host clock) */
     double seconds_eval_device;      /* lattice evaluation, between events on the

@@ -466,17 +466,22 @@ class Octree:
                                             mode, out_val, vp(d_out), vp(stream) if stream else None))
 
     def marching_cubes(self, origin, width, res, iso):
+        """Soup (num_verts, 3) binary32 and stats.  The count call evaluates
+        the lattice; the fill call reuses it, so the lattice times come from
+        the first call and the extraction times from the second."""
         res = np.asarray(res, np.int32)
         st = McStats()
         check(lib().its_mc_extract(self.ctx.handle, self._h, _f64(origin), _f64(width), res, iso, None, 0,
                                    C.byref(st)))
         n = st.num_verts
+        lat = (st.seconds_eval, st.seconds_eval_device)
         tri = np.zeros((max(n, 1), 3), np.float32)
-        check(lib().its_mc_extract(self.ctx.handle, self._h, _f64(origin), _f64(width), res, iso,
-                                   tri.ctypes.data_as(vp), n, C.byref(st)))
+        if n:
+            check(lib().its_mc_extract(self.ctx.handle, self._h, _f64(origin), _f64(width), res, iso,
+                                       tri.ctypes.data_as(vp), n, C.byref(st)))
         return tri[:n], {"num_verts": st.num_verts, "num_valid_voxels": st.num_valid_voxels,
-                         "seconds_eval": st.seconds_eval, "seconds_extract": st.seconds_extract,
-                         "seconds_eval_device": st.seconds_eval_device,
+                         "seconds_eval": lat[0], "seconds_extract": st.seconds_extract,
+                         "seconds_eval_device": lat[1],
                          "seconds_extract_device": st.seconds_extract_device}
 
     def mc_lattice(self, origin, width, res):

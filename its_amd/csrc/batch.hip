@@ -310,6 +310,7 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
     for (int q = 0; q < ns; ++q) {
         its_svo s = b->svos[small[q]];
         Fit* f = fit_get(s);
+        lambda_changing(s);
         s->lambda64.ensure(f->n);
         FitDesc& D = h[q];
         D = FitDesc{};
@@ -376,6 +377,7 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
             order[q] = q;
             its_svo s = b->svos[big[q]];
             (void)fit_get(s);   // solver state built here, on the context stream
+            lambda_changing(s);
             s->lambda64.ensure(s->num_verts);
         }
         std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {

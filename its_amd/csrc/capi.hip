@@ -412,6 +412,7 @@ int its_lscg_solve(its_ctx ctx, its_svo s, const double* b, double* lambda_inout
         ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "no right-hand side: pass b or call its_sdf_at_vertices");
         d_b = s->rhs.p;
     }
+    lambda_changing(s);
     s->lambda64.ensure(n);
     if (warm) ITS_HIP(hipMemcpyAsync(s->lambda64.p, lambda_inout, n * sizeof(double), hipMemcpyHostToDevice, st));
     lscg_solve(s, d_b, s->lambda64.p, tol, max_iter, warm != 0, stats);
@@ -442,6 +443,7 @@ int its_svo_set_lambda(its_ctx ctx, its_svo s, const double* lambda) {
     ITS_REQUIRE(ctx == s->ctx && lambda, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     const int64_t n = s->num_verts;
+    lambda_changing(s);
     s->lambda64.ensure(std::max<int64_t>(n, 1));
     if (n) ITS_HIP(hipMemcpyAsync(s->lambda64.p, lambda, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     s->has_lambda = true;
@@ -844,6 +846,7 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, dou
         d_b = db.p;
     }
     ITS_HIP(hipEventRecord(ctx->ev0, st));
+    lambda_changing(s);
     s->lambda64.ensure(n);
     DBuf<uint16_t> x(16 * n);
     DBuf<float> scale(n), y(n);

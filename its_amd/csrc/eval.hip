@@ -752,6 +752,7 @@ void launch_eval(Loader ld, its_svo s, int64_t N, const EvalParams& P, float* ou
 }  // namespace
 
 void field_build(its_svo s) {
+    lambda_changing(s);
     ITS_REQUIRE(s->has_lambda, ITS_E_STATE, "no coefficients");
     ITS_REQUIRE(s->depth > 0, ITS_E_STATE, "empty octree");
     hipStream_t st = s->ctx->stream;

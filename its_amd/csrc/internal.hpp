@@ -73,6 +73,15 @@ struct its_svo_s {
     its::DBuf<float4> dense_blocks;   // per dense_level cell: corner values of levels >= dense_level,
                                       // child link in the 5 low bits of each
     int dense_level = 0;
+    // bumped before anything may change lambda64 or the evaluation field;
+    // a kept marching-cubes lattice is reused only at the same version
+    uint64_t lambda_version = 0;
+    struct {
+        bool valid;
+        uint64_t version;
+        double origin[3], width[3];
+        int res[3];
+    } mc_key{};
     // grow-only work space: two binary64 levels of the dense cascade, and the
     // marching-cubes lattice, cube cases, per-step vertex counts and soup
     its::DBuf<double> casc[2];
@@ -85,6 +94,8 @@ struct its_svo_s {
 };
 
 namespace its {
+// call before writing lambda64 or rebuilding the field
+inline void lambda_changing(its_svo s) { ++s->lambda_version; }
 // tunables (capi.hip)
 int64_t tune_get(int key);
 // svo.hip

@@ -200,8 +200,34 @@ void upload_tables() {
 // The octree lattice spacing is width.x / R on all axes, and the MC spacing is
 // width[a] / res; the float casts of the box padding make these differ by
 // O(1e-8) relative, a difference in f far below binary32 resolution.
+static void lattice_compute(its_svo s, const double origin[3], const double width[3], const int res[3],
+                            DBuf<float>& lat, hipStream_t st);
+
+// The last lattice stays in the octree's work space and is reused while the
+// grid and the coefficients are the same: the count and fill calls of one
+// extraction, and extractions at several iso values (mcVisualization).
 static void lattice_values(its_svo s, const double origin[3], const double width[3], const int res[3],
                            DBuf<float>& lat, hipStream_t st) {
+    auto& key = s->mc_key;
+    bool same = key.valid && key.version == s->lambda_version && &lat == &s->mc_lat;
+    for (int a = 0; a < 3; ++a)
+        same = same && key.origin[a] == origin[a] && key.width[a] == width[a] && key.res[a] == res[a];
+    if (same) return;
+    key.valid = false;
+    lattice_compute(s, origin, width, res, lat, st);
+    if (&lat == &s->mc_lat) {
+        key.valid = true;
+        key.version = s->lambda_version;
+        for (int a = 0; a < 3; ++a) {
+            key.origin[a] = origin[a];
+            key.width[a] = width[a];
+            key.res[a] = res[a];
+        }
+    }
+}
+
+static void lattice_compute(its_svo s, const double origin[3], const double width[3], const int res[3],
+                            DBuf<float>& lat, hipStream_t st) {
     double vs[3] = {width[0] / res[0], width[1] / res[1], width[2] / res[2]};
     int64_t dims[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
     bool aligned = s->depth > 0;

@@ -99,3 +99,33 @@ def test_mc_ragged_grids(ctx, res):
         assert st["num_valid_voxels"] == nvalid_r and st["num_verts"] == len(soup_r)
         if len(soup_r):
             assert np.abs(tri.astype(np.float64) - soup_r).max() <= 1e-4 * c.width[0]
+
+
+@pytest.mark.parametrize("res", [48, 64])
+def test_mc_lattice_follows_coefficients(ctx, res):
+    """The octree keeps its last marching-cubes lattice for the next call on
+    the same grid; new coefficients must not reuse it (res 64 is the octree's
+    own lattice, the dense cascade; res 48 goes through the field walk)."""
+    F = fitted("icosphere", 64)
+    c = F.case
+    print(f"[mc] lattice cache res={res}", flush=True)
+    g = _gpu(ctx, F)
+    R3 = (res, res, res)
+    ax = [c.origin[a] + np.arange(res + 1) * (c.width[a] / res) for a in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    lat1 = g.mc_lattice(c.origin, c.width, R3)
+    tri1, st1 = g.marching_cubes(c.origin, c.width, R3, 0.0)
+    tri1b, _ = g.marching_cubes(c.origin, c.width, R3, 0.0)   # kept lattice
+    np.testing.assert_array_equal(tri1, tri1b)
+    lam2 = F.lam + 0.01
+    g.set_lambda(lam2)
+    lat2 = g.mc_lattice(c.origin, c.width, R3)
+    assert np.abs(lat2 - F.svo.eval_dense_fast(lam2, P)).max() <= 1e-5
+    assert np.abs(lat1 - F.svo.eval_dense_fast(F.lam, P)).max() <= 1e-5
+    tri2, st2 = g.marching_cubes(c.origin, c.width, R3, 0.0)
+    assert st2["num_verts"] != st1["num_verts"] or not np.array_equal(tri1, tri2)
+    # a solve writes the coefficients too
+    lam3, _ = g.lscg(F.b, tol=1e-6)
+    lat3 = g.mc_lattice(c.origin, c.width, R3)
+    assert np.abs(lat3 - F.svo.eval_dense_fast(lam3, P)).max() <= 1e-5
