@@ -343,7 +343,14 @@ __global__ __launch_bounds__(kCoarseT) void k_prolong_coarse(Coarse C, const uin
             if (upd) { pi = z[i] + beta * pi; p[i] = pi; }
             uint32_t k0 = ptr[i], k1 = ptr[i + 1];
             float s = 0.f;
-            for (uint32_t k = k0; k < k1; ++k) s += sg[idx[k] - base];
+            for (uint32_t kb = k0; kb < k1; kb += 8) {
+                uint32_t id[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) id[q] = (kb + q < k1) ? idx[kb + q] : base;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (kb + q < k1) s += sg[id[q] - base];
+            }
             float w = (k1 > k0) ? 1.f / (float)(k1 - k0) : 0.f;
             float v = pi + w * s;
             sg[i - base] = v;
@@ -360,11 +367,20 @@ __global__ __launch_bounds__(kCoarseT) void k_restrict_coarse(Coarse C, const ui
     for (int d = C.lc; d < C.L; ++d) {
         for (uint32_t c = C.off[d] + threadIdx.x; c < C.off[d + 1]; c += blockDim.x) {
             float s = 0.f;
-            for (uint32_t k = ptr[c]; k < ptr[c + 1]; ++k) {
-                uint32_t v = ridx[k];
-                uint32_t y = v & 0x0FFFFFFFu;
-                float hy = (d == C.lc) ? h[y] : sh[y - base];
-                s += (1.f / (float)(1u << (v >> 28))) * hy;
+            const uint32_t k0 = ptr[c], k1 = ptr[c + 1];
+            for (uint32_t kb = k0; kb < k1; kb += 8) {
+                uint32_t v[8];
+                float hy[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : base;   // base: in bounds for both reads
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t y = v[q] & 0x0FFFFFFFu;
+                    hy[q] = (kb + q < k1) ? ((d == C.lc) ? h[y] : sh[y - base]) : 0.f;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (kb + q < k1) s += ldexpf(hy[q], -(int)(v[q] >> 28));
             }
             float hc = h[c] + s;
             sh[c - base] = hc;
