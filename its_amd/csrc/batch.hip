@@ -60,8 +60,7 @@ __device__ void prolong64(const FitDesc& D, const double* x, double* g) {
     for (int d = D.L - 1; d >= 0; --d) {
         for (int64_t i = D.off[d] + threadIdx.x; i < D.off[d + 1]; i += kB) {
             const uint32_t k0 = D.pro_ptr[i], k1 = D.pro_ptr[i + 1];
-            double s = 0;
-            for (uint32_t k = k0; k < k1; ++k) s += g[D.pro_idx[k]];
+            const double s = gather_parents(D.pro_idx, k0, k1, (const double*)g);
             const double w = (k1 > k0) ? 1.0 / (double)(k1 - k0) : 0.0;
             g[i] = x[i] + w * s;
         }
@@ -83,12 +82,7 @@ __device__ void apply_AT(const FitDesc& D, const T* r, T* h) {
     __syncthreads();
     for (int d = 1; d < D.L; ++d) {
         for (int64_t c = D.off[d] + threadIdx.x; c < D.off[d + 1]; c += kB) {
-            T s = 0;
-            for (uint32_t k = D.res_ptr[c]; k < D.res_ptr[c + 1]; ++k) {
-                const uint32_t v = D.res_idx[k];
-                s += ((T)1 / (T)(1u << (v >> 28))) * h[v & 0x0FFFFFFFu];
-            }
-            h[c] += s;
+            h[c] += gather_children(D.res_idx, D.res_ptr[c], D.res_ptr[c + 1], (const T*)h);
         }
         __syncthreads();
     }
@@ -143,8 +137,7 @@ __device__ int64_t inner_cgls(const FitDesc& D, double rel_tol, int64_t max_it, 
                 const float pi = D.z[i] + (float)beta * D.p[i];
                 D.p[i] = pi;
                 const uint32_t k0 = D.pro_ptr[i], k1 = D.pro_ptr[i + 1];
-                float s = 0;
-                for (uint32_t k = k0; k < k1; ++k) s += D.g[D.pro_idx[k]];
+                const float s = gather_parents(D.pro_idx, k0, k1, (const float*)D.g);
                 const float w = (k1 > k0) ? 1.f / (float)(k1 - k0) : 0.f;
                 D.g[i] = pi + w * s;
             }
@@ -178,12 +171,7 @@ __device__ int64_t inner_cgls(const FitDesc& D, double rel_tol, int64_t max_it, 
         __syncthreads();
         for (int d = 1; d < D.L; ++d) {
             for (int64_t cc = D.off[d] + threadIdx.x; cc < D.off[d + 1]; cc += kB) {
-                float s = 0;
-                for (uint32_t k = D.res_ptr[cc]; k < D.res_ptr[cc + 1]; ++k) {
-                    const uint32_t v = D.res_idx[k];
-                    s += (1.f / (float)(1u << (v >> 28))) * D.h[v & 0x0FFFFFFFu];
-                }
-                D.h[cc] += s;
+                D.h[cc] += gather_children(D.res_idx, D.res_ptr[cc], D.res_ptr[cc + 1], (const float*)D.h);
             }
             __syncthreads();
         }

@@ -9,6 +9,46 @@ struct Coarse {
     uint32_t off[kMaxLevels + 1];   // global vertex offsets
 };
 
+// Operator gathers shared by the single-mesh and the batched solver.  Index
+// lists are read 8 entries at a time and then the 8 values, so a thread
+// waits for two loads per 8 entries instead of two per entry; the sums keep
+// the list order.
+// sum of g over the prolongation list [k0, k1) (coarse parents of a vertex)
+template <class T>
+__device__ inline T gather_parents(const uint32_t* idx, uint32_t k0, uint32_t k1, const T* g) {
+    T s = 0;
+    for (uint32_t kb = k0; kb < k1; kb += 8) {
+        uint32_t id[8];
+        T gv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) id[q] = (kb + q < k1) ? idx[kb + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) gv[q] = (kb + q < k1) ? g[id[q]] : (T)0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (kb + q < k1) s += gv[q];
+    }
+    return s;
+}
+// sum of 2^-k h over the restriction list [k0, k1) (fine children, with
+// log2 of the child's parent count k in bits 28 and up), weights exact
+template <class T>
+__device__ inline T gather_children(const uint32_t* ridx, uint32_t k0, uint32_t k1, const T* h) {
+    T s = 0;
+    for (uint32_t kb = k0; kb < k1; kb += 8) {
+        uint32_t v[8];
+        T hv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) hv[q] = (kb + q < k1) ? h[v[q] & 0x0FFFFFFFu] : (T)0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (kb + q < k1) s += ldexp(hv[q], -(int)(v[q] >> 28));
+    }
+    return s;
+}
+
 struct Fit {
     int64_t n = 0;
     int L = 0;

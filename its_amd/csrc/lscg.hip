@@ -215,21 +215,8 @@ __global__ __launch_bounds__(kT) void k_prolong(int64_t b, int64_t e, const uint
         pi = z[i] + (T)scal[S_BETA] * pi;
         p[i] = pi;
     }
-    uint32_t k0 = ptr[i], k1 = ptr[i + 1];
-    // the (at most 8) parent indices, then their values, are loaded as
-    // batches; the sum keeps the list order
-    T s = 0;
-    for (uint32_t kb = k0; kb < k1; kb += 8) {
-        uint32_t id[8];
-        T gv[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) id[q] = (kb + q < k1) ? idx[kb + q] : 0u;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) gv[q] = (kb + q < k1) ? g[id[q]] : (T)0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (kb + q < k1) s += gv[q];
-    }
+    const uint32_t k0 = ptr[i], k1 = ptr[i + 1];
+    const T s = gather_parents(idx, k0, k1, (const T*)g);
     T w = (k1 > k0) ? (T)1 / (T)(k1 - k0) : (T)0;
     g[i] = pi + w * s;
 }
@@ -307,22 +294,7 @@ __global__ __launch_bounds__(kT) void k_restrict(int64_t b, int64_t e, const uin
     if (check && ctl[0]) return;
     int64_t c = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= e) return;
-    // children in batches of 8: indices, then values, then the sum in list
-    // order; the weight 2^-k is applied exactly with ldexp
-    const uint32_t k0 = ptr[c], k1 = ptr[c + 1];
-    T s = 0;
-    for (uint32_t kb = k0; kb < k1; kb += 8) {
-        uint32_t v[8];
-        T hv[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : 0u;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) hv[q] = (kb + q < k1) ? h[v[q] & 0x0FFFFFFFu] : (T)0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (kb + q < k1) s += ldexp(hv[q], -(int)(v[q] >> 28));
-    }
-    h[c] += s;
+    h[c] += gather_children(ridx, ptr[c], ptr[c + 1], (const T*)h);
 }
 
 // The coarse levels [lc, L) hold few vertices (at most kCoarse).  One
