@@ -252,6 +252,11 @@ struct its_batch_s {
     hipStream_t work[kWorkers] = {};
     std::vector<its::FitDesc> host;   // the descriptors the graph was captured with
     ~its_batch_s() {
+        // no stream may still use the buffers freed below
+        if (side) (void)hipStreamSynchronize(side);
+        for (hipStream_t w : work)
+            if (w) (void)hipStreamSynchronize(w);
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (side) (void)hipStreamDestroy(side);
@@ -343,6 +348,7 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
             b->gexec = nullptr;
             b->graph = nullptr;
             b->descs.alloc(ns);
+            ITS_HIP(hipStreamSynchronize(st));   // the allocation is ordered on the context stream
             ITS_HIP(hipMemcpyAsync(b->descs.p, h.data(), ns * sizeof(FitDesc), hipMemcpyHostToDevice, b->side));
             ITS_HIP(hipStreamSynchronize(b->side));
             ITS_HIP(hipStreamBeginCapture(b->side, hipStreamCaptureModeThreadLocal));
@@ -401,7 +407,11 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (seconds) *seconds = secs;
     std::vector<double> hs(8 * (size_t)std::max(ns, 1));
-    if (ns > 0) ITS_HIP(hipMemcpy(hs.data(), b->stats.p, 8 * (size_t)ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (ns > 0) {
+        ITS_HIP(hipMemcpyAsync(hs.data(), b->stats.p, 8 * (size_t)ns * sizeof(double), hipMemcpyDeviceToHost,
+                               b->ctx->stream));
+        ITS_HIP(hipStreamSynchronize(b->ctx->stream));
+    }
     for (int q = 0; q < ns; ++q) {
         if (!out) break;
         its_lscg_stats& S = out[small[q]];

@@ -87,6 +87,13 @@ int its_ctx_create(int device, its_ctx* out) {
     ITS_HIP(hipEventCreate(&c->ev1));
     for (hipEvent_t& e : c->mc_ev) ITS_HIP(hipEventCreate(&e));
     ITS_HIP(hipGetDeviceProperties(&c->prop, device));
+    // stream-ordered allocations keep freed blocks in the device's default
+    // pool instead of returning them at every synchronisation
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     *out = c;
     ITS_API_END
 }
@@ -125,6 +132,7 @@ int its_ctx_synchronize(its_ctx ctx) {
     ITS_API_BEGIN
     ITS_REQUIRE(ctx, ITS_E_INVALID, "null context");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     ITS_HIP(hipStreamSynchronize(ctx->stream));
     ITS_API_END
 }
@@ -194,6 +202,7 @@ static int svo_build_common(its_ctx ctx, const double* tris, bool device_ptr, in
     ITS_REQUIRE(T >= 0 && (T == 0 || tris), ITS_E_INVALID, "bad triangle array");
     ITS_REQUIRE(R >= 2 && R <= 1024 && (R & (R - 1)) == 0, ITS_E_RANGE, "R must be a power of two in [2, 1024]");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     std::unique_ptr<its_svo_s> s(new its_svo_s());
     s->ctx = ctx;
     s->R = R;
@@ -224,6 +233,7 @@ int its_svo_destroy(its_svo svo) {
     ITS_API_BEGIN
     if (!svo) return ITS_OK;
     DeviceGuard g(svo->ctx->device);
+    AllocStream as(svo->ctx->stream);
     (void)hipStreamSynchronize(svo->ctx->stream);
     delete svo;
     ITS_API_END
@@ -254,6 +264,7 @@ int its_svo_export_nodes(its_svo s, uint32_t* morton, uint8_t* is_leaf, double* 
     ITS_API_BEGIN
     check_svo(s);
     DeviceGuard g(s->ctx->device);
+    AllocStream as(s->ctx->stream);
     hipStream_t st = s->ctx->stream;
     const int64_t n = s->num_nodes;
     if (n == 0) return ITS_OK;
@@ -280,6 +291,7 @@ int its_svo_export_vert_lattice(its_svo s, uint16_t* ijk, uint8_t* level) {
     ITS_API_BEGIN
     check_svo(s);
     DeviceGuard g(s->ctx->device);
+    AllocStream as(s->ctx->stream);
     hipStream_t st = s->ctx->stream;
     const int64_t n = s->num_verts;
     if (n == 0) return ITS_OK;
@@ -306,6 +318,7 @@ int its_svo_export_fine_edges(its_svo s, double* ends, uint32_t* owner, int64_t 
     ITS_API_BEGIN
     check_svo(s);
     DeviceGuard g(s->ctx->device);
+    AllocStream as(s->ctx->stream);
     hipStream_t st = s->ctx->stream;
     DBuf<double> E;
     DBuf<uint32_t> O;
@@ -324,6 +337,7 @@ int its_svo_export_verts(its_svo s, double* xyz, uint32_t* owner) {
     ITS_API_BEGIN
     check_svo(s);
     DeviceGuard g(s->ctx->device);
+    AllocStream as(s->ctx->stream);
     hipStream_t st = s->ctx->stream;
     const int64_t n = s->num_verts;
     if (n == 0) return ITS_OK;
@@ -337,6 +351,7 @@ int its_svo_export_voxels(its_svo s, uint32_t* morton) {
     ITS_API_BEGIN
     check_svo(s);
     DeviceGuard g(s->ctx->device);
+    AllocStream as(s->ctx->stream);
     if (s->num_voxels && morton) {
         ITS_HIP(hipMemcpyAsync(morton, s->lv[0].occ.p, s->num_voxels * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                s->ctx->stream));
@@ -352,6 +367,7 @@ int its_sdf_at_vertices(its_ctx ctx, its_svo s, const double* V, int64_t nV, con
     ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
     ITS_REQUIRE(V && F && nV > 0 && nF > 0, ITS_E_INVALID, "empty mesh");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     for (int64_t i = 0; i < 3 * nF; ++i)
         ITS_REQUIRE(F[i] >= 0 && F[i] < nV, ITS_E_INVALID, "face index out of range");
@@ -379,6 +395,7 @@ int its_signed_distance(its_ctx ctx, const double* V, int64_t nV, const int32_t*
     for (int64_t i = 0; i < 3 * nF; ++i)
         ITS_REQUIRE(F[i] >= 0 && F[i] < nV, ITS_E_INVALID, "face index out of range");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     if (N == 0) return ITS_OK;
     DBuf<double> dV = upload(V, 3 * (size_t)nV, st);
@@ -400,6 +417,7 @@ int its_lscg_solve(its_ctx ctx, its_svo s, const double* b, double* lambda_inout
     ITS_REQUIRE(lambda_inout, ITS_E_INVALID, "null lambda");
     ITS_REQUIRE(tol >= 0, ITS_E_INVALID, "negative tolerance");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     const int64_t n = s->num_verts;
     ITS_REQUIRE(n > 0, ITS_E_STATE, "octree has no vertices");
@@ -428,6 +446,7 @@ int its_collocation_mul(its_ctx ctx, its_svo s, const double* x, double* y, int 
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && x && y, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     const int64_t n = s->num_verts;
     DBuf<double> dx = upload(x, n, st), dy(n);
@@ -442,6 +461,7 @@ int its_svo_set_lambda(its_ctx ctx, its_svo s, const double* lambda) {
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && lambda, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     const int64_t n = s->num_verts;
     lambda_changing(s);
     s->lambda64.ensure(std::max<int64_t>(n, 1));
@@ -457,6 +477,7 @@ int its_svo_get_lambda(its_svo s, double* lambda) {
     ITS_REQUIRE(lambda, ITS_E_INVALID, "null lambda");
     ITS_REQUIRE(s->has_lambda, ITS_E_STATE, "no coefficients");
     DeviceGuard g(s->ctx->device);
+    AllocStream as(s->ctx->stream);
     ITS_HIP(hipMemcpyAsync(lambda, s->lambda64.p, s->num_verts * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
     ITS_HIP(hipStreamSynchronize(s->ctx->stream));
     ITS_API_END
@@ -474,6 +495,7 @@ int its_bspline_eval(its_ctx ctx, its_svo s, const float* pts, int64_t N, const 
     check_mode(mode);
     ITS_REQUIRE(ctx == s->ctx && (N == 0 || (pts && f)) && box_end, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     if (N == 0) return ITS_OK;
     std::vector<float> soa(3 * (size_t)N);
@@ -496,6 +518,7 @@ int its_bspline_eval_f64pts(its_ctx ctx, its_svo s, const double* pts, int64_t N
     check_mode(mode);
     ITS_REQUIRE(ctx == s->ctx && (N == 0 || (pts && f)) && box_end, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     if (N == 0) return ITS_OK;
     DBuf<double> d = upload(pts, 3 * (size_t)N, st);
@@ -513,6 +536,7 @@ int its_bspline_eval_device(its_ctx ctx, its_svo s, const float* d_x, const floa
     check_mode(mode);
     ITS_REQUIRE(ctx == s->ctx && box_end && (N == 0 || (d_x && d_y && d_z && d_f)), ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     eval_points(s, d_x, d_y, d_z, N, box_end, mode, out_val, d_f, stream ? (hipStream_t)stream : ctx->stream);
     ITS_API_END
 }
@@ -534,6 +558,7 @@ int its_mc_extract(its_ctx ctx, its_svo s, const double origin[3], const double 
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && origin && width && res, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mc_extract(s, origin, width, res, iso, tri_xyz, capacity, stats);
     ITS_API_END
 }
@@ -544,6 +569,7 @@ int its_mc_lattice(its_ctx ctx, its_svo s, const double origin[3], const double 
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && origin && width && res && values, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mc_lattice(s, origin, width, res, values);
     ITS_API_END
 }
@@ -553,6 +579,7 @@ int its_lscg_diag(its_ctx ctx, its_svo s, double* invd) {
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && invd, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     const int64_t n = s->num_verts;
     if (n == 0) return ITS_OK;
@@ -569,6 +596,7 @@ int its_lattice_f64(its_ctx ctx, its_svo s, double* values) {
     ITS_REQUIRE(ctx == s->ctx && values, ITS_E_INVALID, "bad arguments");
     ITS_REQUIRE(s->has_lambda && s->depth > 0, ITS_E_STATE, "no coefficients");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     DBuf<double> lat;
     dense_cascade64(s, 0, lat);
@@ -611,6 +639,7 @@ int its_shell_points(its_ctx ctx, its_svo s, const double* V, int64_t nV, const 
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     ShellInput in = shell_upload(V, nV, F, nF, st);
     DBuf<double> P;
@@ -633,6 +662,7 @@ int its_shell_iso(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int
     ITS_REQUIRE(ctx == s->ctx && iso, ITS_E_INVALID, "bad arguments");
     ITS_REQUIRE(s->field_ready, ITS_E_STATE, "no coefficient field: run a fit or set lambda first");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     ShellInput in = shell_upload(V, nV, F, nF, st);
     DBuf<double> P;
@@ -664,6 +694,7 @@ int its_batch_destroy(its_batch b) {
     ITS_API_BEGIN
     if (b) {
         DeviceGuard g(batch_ctx(b)->device);
+    AllocStream as(batch_ctx(b)->stream);
         batch_delete(b);
     }
     ITS_API_END
@@ -680,6 +711,7 @@ int its_batch_fit(its_batch b, double tol, int64_t max_iter, int warm, its_lscg_
     ITS_REQUIRE(b, ITS_E_INVALID, "null batch");
     ITS_REQUIRE(tol > 0, ITS_E_INVALID, "tolerance must be positive");
     DeviceGuard g(batch_ctx(b)->device);
+    AllocStream as(batch_ctx(b)->stream);
     batch_fit(b, tol, max_iter, warm != 0, stats, seconds);
     ITS_API_END
 }
@@ -687,6 +719,7 @@ int its_batch_fields(its_batch b) {
     ITS_API_BEGIN
     ITS_REQUIRE(b, ITS_E_INVALID, "null batch");
     DeviceGuard g(batch_ctx(b)->device);
+    AllocStream as(batch_ctx(b)->stream);
     batch_fields(b);
     ITS_API_END
 }
@@ -704,6 +737,7 @@ int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const
     ITS_REQUIRE(offsets[count] == 0 || (d_x && d_y && d_z && d_f), ITS_E_INVALID, "null query arrays");
     its_ctx ctx = batch_ctx(b);
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     std::vector<its_svo> v(count);
     for (int64_t m = 0; m < count; ++m) v[m] = batch_svo(b, m);
     ITS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
@@ -723,6 +757,7 @@ int its_mlp_create(its_ctx ctx, uint64_t seed, its_mlp* out) {
     ITS_API_BEGIN
     ITS_REQUIRE(ctx && out, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     its_mlp net = mlp_new();
     try {
         mlp_init(mlp_of(net), ctx, seed);
@@ -737,6 +772,7 @@ int its_mlp_destroy(its_mlp net) {
     ITS_API_BEGIN
     if (net) {
         DeviceGuard g(mlp_ctx(net)->device);
+    AllocStream as(mlp_ctx(net)->stream);
         mlp_delete(net);
     }
     ITS_API_END
@@ -747,6 +783,7 @@ int its_mlp_set_params(its_mlp net, const float* params) {
     ITS_REQUIRE(params, ITS_E_INVALID, "null parameters");
     its_ctx ctx = mlp_ctx(net);
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mlp_set(net, params, ctx->stream);
     ITS_API_END
 }
@@ -756,6 +793,7 @@ int its_mlp_get_params(its_mlp net, float* params) {
     ITS_REQUIRE(params, ITS_E_INVALID, "null parameters");
     its_ctx ctx = mlp_ctx(net);
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mlp_get(net, params, ctx->stream);
     ITS_API_END
 }
@@ -765,6 +803,7 @@ int its_mlp_forward_device(its_mlp net, const uint16_t* d_x, int64_t rows, float
     ITS_REQUIRE(rows >= 0 && (rows == 0 || (d_x && d_y)), ITS_E_INVALID, "bad arguments");
     its_ctx ctx = mlp_ctx(net);
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mlp_forward(mlp_of(net), d_x, rows, d_y, stream ? (hipStream_t)stream : ctx->stream);
     ITS_API_END
 }
@@ -774,6 +813,7 @@ int its_mlp_grad_device(its_mlp net, const uint16_t* d_x, const float* d_t, int6
     ITS_REQUIRE(rows > 0 && d_x && d_t, ITS_E_INVALID, "bad arguments");
     its_ctx ctx = mlp_ctx(net);
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mlp_grad(mlp_of(net), d_x, d_t, rows, ctx->stream);
     mlp_read_grad(net, grad, loss, rows, ctx->stream);
     ITS_API_END
@@ -784,6 +824,7 @@ int its_mlp_train_step(its_mlp net, const uint16_t* d_x, const float* d_t, int64
     ITS_REQUIRE(rows > 0 && d_x && d_t && lr > 0, ITS_E_INVALID, "bad arguments");
     its_ctx ctx = mlp_ctx(net);
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     mlp_grad(mlp_of(net), d_x, d_t, rows, ctx->stream);
     mlp_adam(mlp_of(net), lr, ctx->stream);
     if (loss) mlp_read_grad(net, nullptr, loss, rows, ctx->stream);
@@ -794,6 +835,7 @@ int its_warm_features(its_ctx ctx, its_svo s, uint16_t* d_x, float* d_scale, voi
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && d_x && d_scale, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     warm_features(s, d_x, d_scale, stream ? (hipStream_t)stream : ctx->stream);
     ITS_API_END
 }
@@ -802,6 +844,7 @@ int its_warm_dual(its_ctx ctx, its_svo s, const double* lambda, double* w) {
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && lambda && w, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     const int64_t n = s->num_verts;
     if (n == 0) return ITS_OK;
@@ -817,6 +860,7 @@ int its_warm_apply(its_ctx ctx, its_svo s, const double* w, double* lambda0) {
     check_svo(s);
     ITS_REQUIRE(ctx == s->ctx && w && lambda0, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     const int64_t n = s->num_verts;
     if (n == 0) return ITS_OK;
@@ -836,6 +880,7 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, dou
     ITS_REQUIRE(tol > 0, ITS_E_INVALID, "tolerance must be positive");
     ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "the features need its_sdf_at_vertices");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     const int64_t n = s->num_verts;
     ITS_REQUIRE(n > 0, ITS_E_STATE, "empty octree");
@@ -892,6 +937,7 @@ int its_timer_start(its_ctx ctx) {
     ITS_API_BEGIN
     ITS_REQUIRE(ctx, ITS_E_INVALID, "null context");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     ITS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     ITS_API_END
 }
@@ -899,6 +945,7 @@ int its_timer_stop(its_ctx ctx, double* ms) {
     ITS_API_BEGIN
     ITS_REQUIRE(ctx && ms, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
     ITS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     ITS_HIP(hipEventSynchronize(ctx->ev1));
     float t = 0;

@@ -34,8 +34,25 @@ void set_last_error(const std::string& msg);
 #define ITS_LAUNCH_CHECK() ITS_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------- buffers ---
-// Device allocation owned by a host object.  Allocations go through
-// hipMallocAsync-free plain hipMalloc so buffers outlive streams.
+// Device allocation owned by a host object.
+// Stream for stream-ordered allocation on this thread (the entry points set
+// their context's stream).  When set, DBuf allocates and frees with
+// hipMallocAsync / hipFreeAsync on it, so temporaries cost no device-wide
+// synchronisation (hipFree waits for the whole device); when not set, plain
+// hipMalloc / hipFree.  Such memory is used on other streams only after a
+// synchronisation, and may be freed with hipFree once its stream is gone.
+inline hipStream_t& alloc_stream() {
+    thread_local hipStream_t s = nullptr;
+    return s;
+}
+struct AllocStream {
+    hipStream_t prev;
+    explicit AllocStream(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+    ~AllocStream() { alloc_stream() = prev; }
+    AllocStream(const AllocStream&) = delete;
+    AllocStream& operator=(const AllocStream&) = delete;
+};
+
 template <class T>
 struct DBuf {
     T* p = nullptr;
@@ -54,7 +71,8 @@ struct DBuf {
         release();
         n = count;
         if (count) {
-            hipError_t e = hipMalloc(&p, count * sizeof(T));
+            hipStream_t s = alloc_stream();
+            hipError_t e = s ? hipMallocAsync((void**)&p, count * sizeof(T), s) : hipMalloc(&p, count * sizeof(T));
             if (e != hipSuccess) {
                 p = nullptr;
                 n = 0;
@@ -66,7 +84,11 @@ struct DBuf {
     // grow-only reuse
     void ensure(size_t count) { if (count > n) alloc(count); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            hipStream_t s = alloc_stream();
+            if (s) (void)hipFreeAsync(p, s);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }

@@ -89,6 +89,7 @@ void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st) {
         S.off[d] = (uint32_t)s->esum_verts[d];
         S.width[d] = s->voxel_width * (double)(1u << d);
     }
+    AllocStream as(st);   // the flag lives on the stream that uses it
     DBuf<int> err(1);
     ITS_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_features, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, s->vx.p, s->vy.p, s->vz.p,
