@@ -44,7 +44,7 @@ constexpr int kT = 256;
 constexpr int kRedBlocks = 512;  // fixed partial count for deterministic sums
 
 // device scalar slots
-enum Scal { S_ABSNEW = 0, S_ALPHA, S_BETA, S_TT, S_RN2, S_THRESH, S_RN2_0, S_ZZ, S_COUNT };
+enum Scal { S_ABSNEW = 0, S_ALPHA, S_BETA, S_TT, S_RN2, S_THRESH, S_RN2_0, S_ZZ, S_ZZ2, S_COUNT };
 
 // ---------------------------------------------------------------- structure
 __global__ void k_pro_count(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
@@ -585,6 +585,24 @@ double dot64(Fit* f, const double* a, const double* b, hipStream_t st) {
     ITS_LAUNCH_CHECK();
     return read1(f->scal.p + S_ZZ, st);
 }
+// two dot products, read back together
+void dot64_pair(Fit* f, const double* a1, const double* b1, const double* a2, const double* b2, hipStream_t st,
+                double& r1, double& r2) {
+    unsigned nb = red_blocks(f->n);
+    hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kT), 0, st, f->n, a1, b1, f->part.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p + S_ZZ);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kT), 0, st, f->n, a2, b2, f->part.p);
+    ITS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p + S_ZZ2);
+    ITS_LAUNCH_CHECK();
+    double v[2];
+    ITS_HIP(hipMemcpyAsync(v, f->scal.p + S_ZZ, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    r1 = v[0];
+    r2 = v[1];
+}
 
 // binary32 transposed pass on f->r into f->h (with optional fused update)
 void launch_AT32(Fit* f, hipStream_t st, int do_update, int check) {
@@ -777,11 +795,12 @@ void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int6
     if (!warm) ITS_HIP(hipMemsetAsync(d_lambda, 0, n * sizeof(double), st));
     // ||A^T b||^2
     apply_AT64(f, d_b, f->h64.p, st);
-    double rhs2 = dot64(f, f->h64.p, f->h64.p, st);
-    double bb = dot64(f, d_b, d_b, st);
+    double rhs2 = 0, bb = 0;
+    dot64_pair(f, f->h64.p, f->h64.p, d_b, d_b, st, rhs2, bb);
     its_lscg_stats S{};
     int64_t iters = 0, outer = 0;
     double rel = 0, rel0 = -1;
+    bool r_current = false;   // r64 = b - A lambda for the current lambda
     if (rhs2 == 0) {
         ITS_HIP(hipMemsetAsync(d_lambda, 0, n * sizeof(double), st));
     } else {
@@ -790,6 +809,7 @@ void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int6
             apply_A64(f, d_lambda, f->tmp64.p, st);
             hipLaunchKernelGGL(k_sub64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, d_b, f->tmp64.p, f->r64.p);
             ITS_LAUNCH_CHECK();
+            r_current = true;
             apply_AT64(f, f->r64.p, f->nr64.p, st);
             double rn2 = dot64(f, f->nr64.p, f->nr64.p, st);
             rel = std::sqrt(rn2 / rhs2);
@@ -802,13 +822,17 @@ void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int6
             hipLaunchKernelGGL(k_axpy_f32_to64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->x.p, d_lambda);
             ITS_LAUNCH_CHECK();
             ++outer;
+            // no inner iteration leaves x = 0, so lambda and r64 are unchanged
+            r_current = (it == 0);
             if (it == 0) break;   // no progress possible
         }
     }
-    // final residuals
-    apply_A64(f, d_lambda, f->tmp64.p, st);
-    hipLaunchKernelGGL(k_sub64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, d_b, f->tmp64.p, f->r64.p);
-    ITS_LAUNCH_CHECK();
+    // final residuals (the loop leaves r64 current except after a zero rhs)
+    if (!r_current) {
+        apply_A64(f, d_lambda, f->tmp64.p, st);
+        hipLaunchKernelGGL(k_sub64, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, d_b, f->tmp64.p, f->r64.p);
+        ITS_LAUNCH_CHECK();
+    }
     double rr = dot64(f, f->r64.p, f->r64.p, st);
     ITS_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
