@@ -19,6 +19,8 @@
 #include <chrono>
 #include <cmath>
 
+#include <mutex>
+
 #include "internal.hpp"
 #include "mc_tables.inc"
 
@@ -176,11 +178,14 @@ __global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, M
     }
 }
 
+// once per device, also when several host threads make their first call
 void upload_tables() {
-    static int dev_done = -1;
+    static std::mutex mu;
+    static uint64_t done_mask = 0;   // devices holding the tables
     int dev;
     ITS_HIP(hipGetDevice(&dev));
-    if (dev_done == dev) return;
+    std::lock_guard<std::mutex> lock(mu);
+    if (dev < 64 && ((done_mask >> dev) & 1u)) return;
     unsigned char nv[256];
     for (int c = 0; c < 256; ++c) {
         int n = 0;
@@ -189,7 +194,7 @@ void upload_tables() {
     }
     ITS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tri), kMcTri, sizeof kMcTri));
     ITS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_nv), nv, sizeof nv));
-    dev_done = dev;
+    if (dev < 64) done_mask |= 1ull << dev;
 }
 }  // namespace
 

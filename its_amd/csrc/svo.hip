@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace its {
@@ -307,12 +309,14 @@ __global__ void k_vxyz(int64_t nv, int64_t voff, const uint16_t* vx, const uint1
     xyz[3 * v + 2] = origin[3 * (int64_t)o + 2] + w * (double)(vz[v] - z);
 }
 
+// once per device, also when several host threads make their first call
 void upload_luts() {
-    static bool done = false;
-    static int dev_done = -1;
+    static std::mutex mu;
+    static uint64_t done_mask = 0;   // devices holding the tables
     int dev;
     ITS_HIP(hipGetDevice(&dev));
-    if (done && dev_done == dev) return;
+    std::lock_guard<std::mutex> lock(mu);
+    if (dev < 64 && ((done_mask >> dev) & 1u)) return;
     uint8_t lp[8 * 27], lc[8 * 27];
     for (int key = 0; key < 8; ++key) {
         int c[3] = {key & 1, (key >> 1) & 1, (key >> 2) & 1};
@@ -332,8 +336,7 @@ void upload_luts() {
     }
     ITS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lut_parent), lp, sizeof lp));
     ITS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lut_child), lc, sizeof lc));
-    done = true;
-    dev_done = dev;
+    if (dev < 64) done_mask |= 1ull << dev;
 }
 
 }  // namespace
