@@ -264,6 +264,9 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo svo, its_mlp net, const double* b, d
 typedef struct its_batch_s* its_batch;
 int its_batch_create(its_ctx ctx, its_batch* out);
 int its_batch_destroy(its_batch b);
+/* svo may come from any context on the batch's device (for instance octrees
+ * built by several host threads, one context each); the fit synchronises
+ * their streams before reading them. */
 int its_batch_add(its_batch b, its_svo svo);
 int its_batch_fit(its_batch b, double tol, int64_t max_iter, int warm, its_lscg_stats* stats, double* seconds);
 int its_batch_fields(its_batch b);

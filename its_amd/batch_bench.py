@@ -4,20 +4,22 @@ Workload (batching): 256 seeded meshes -- noisy icospheres, tori and
 blob unions in turn -- with triangle counts log-uniform in [1k, 200k] and
 octree depth 5-8 (R = 32..256) drawn per mesh; 65,536 uniform queries per mesh.
 
-Stages: octree build and signed-distance right-hand side per mesh; the fit of
+Stages: octree build and signed-distance right-hand side per mesh, from
+several host threads with a context each (--build-threads); the fit of
 all meshes in one persistent launch (its_batch_fit, graph-captured); the
 evaluation fields per mesh with the autotuned collapse bound of its depth;
 the evaluation of all meshes' queries in one launch (its_batch_eval_device).
 The same fits and evaluations also run mesh by mesh through the single-mesh
 entry points for comparison.  Prints one JSON line.
 
-    python -m its_amd.batch_bench [--meshes 256] [--queries 65536]
+    python -m its_amd.batch_bench [--meshes 256] [--queries 65536] [--build-threads 4]
 """
 import argparse
 import json
 import math
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -55,6 +57,8 @@ def main(argv=None):
     ap.add_argument("--queries", type=int, default=65536)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--table", default=os.path.join(HERE, "data", "autotune.json"))
+    ap.add_argument("--build-threads", type=int, default=4,
+                    help="host threads building octrees and right-hand sides, one context each")
     a = ap.parse_args(argv)
     import torch
     torch.cuda.init()
@@ -70,21 +74,50 @@ def main(argv=None):
     specs = [make_mesh(k, rng) for k in range(a.meshes)]
     t_gen = time.perf_counter() - t0
     log(f"[batch] {a.meshes} meshes generated ({t_gen:.1f} s)")
-    items = []
-    t_build = t_rhs = 0.0
-    for k, (kind, V, F, depth) in enumerate(specs):
-        V = _capi.model_to_unit_cube(V)
-        origin, end, width = _capi.uniform_bbox(V)
-        s0 = time.perf_counter()
-        g = _capi.Octree(ctx, meshes.triangles(V, F), origin, width, 1 << depth)
-        s1 = time.perf_counter()
-        b = g.sdf_at_vertices(V, F)
-        s2 = time.perf_counter()
-        t_build += s1 - s0
-        t_rhs += s2 - s1
-        items.append(dict(kind=kind, T=len(F), depth=depth, g=g, b=b, origin=origin, width=width))
-        if (k + 1) % 32 == 0:
-            log(f"[batch] built {k + 1}/{a.meshes} (octrees {t_build:.1f} s, right-hand sides {t_rhs:.1f} s)")
+    # octrees and right-hand sides, from build_threads host threads with a
+    # context (stream) each, meshes taken in order from a shared counter;
+    # the batch below joins octrees of any context on the device
+    nthr = max(1, a.build_threads)
+    bctx = [ctx] if nthr == 1 else [_capi.Context(0) for _ in range(nthr)]
+    items = [None] * a.meshes
+    times = np.zeros((a.meshes, 2))
+    lock = threading.Lock()
+    nxt = [0]
+    errors = []
+
+    def build_worker(t):
+        try:
+            while True:
+                with lock:
+                    k = nxt[0]
+                    nxt[0] += 1
+                if k >= a.meshes:
+                    return
+                kind, V, F, depth = specs[k]
+                V = _capi.model_to_unit_cube(V)
+                origin, end, width = _capi.uniform_bbox(V)
+                s0 = time.perf_counter()
+                g = _capi.Octree(bctx[t], meshes.triangles(V, F), origin, width, 1 << depth)
+                s1 = time.perf_counter()
+                b = g.sdf_at_vertices(V, F)
+                s2 = time.perf_counter()
+                times[k] = (s1 - s0, s2 - s1)
+                items[k] = dict(kind=kind, T=len(F), depth=depth, g=g, b=b, origin=origin, width=width)
+        except Exception as e:   # re-raised below
+            errors.append(e)
+
+    w0 = time.perf_counter()
+    workers = [threading.Thread(target=build_worker, args=(t,)) for t in range(nthr)]
+    for th in workers:
+        th.start()
+    for th in workers:
+        th.join()
+    t_build_wall = time.perf_counter() - w0
+    if errors:
+        raise errors[0]
+    t_build, t_rhs = float(times[:, 0].sum()), float(times[:, 1].sum())
+    log(f"[batch] built {a.meshes} on {nthr} threads in {t_build_wall:.2f} s (summed per-call: octrees "
+        f"{t_build:.2f} s, right-hand sides {t_rhs:.2f} s)")
 
     # fit: mesh by mesh, then all in one launch
     for it in items:
@@ -156,16 +189,20 @@ def main(argv=None):
         "eval_bit_identical": same,
         "all_converged": bool(all(s["converged"] for s in stats)),
         "max_iterations": int(max(s["iterations"] for s in stats)),
-        "build_s": t_build, "rhs_s": t_rhs, "fields_s": t_fields,
+        "build_threads": nthr, "build_rhs_wall_s": t_build_wall,
+        "build_s_summed": t_build, "rhs_s_summed": t_rhs, "fields_s": t_fields,
         "meshes_per_s_fit_eval": a.meshes / gpu_total,
-        "meshes_per_s_end_to_end": a.meshes / (t_build + t_rhs + t_fit + t_fields + t_eval),
-        "latency_ms_per_mesh_end_to_end": (t_build + t_rhs + t_fit + t_fields + t_eval) / a.meshes * 1e3,
+        "meshes_per_s_end_to_end": a.meshes / (t_build_wall + t_fit + t_fields + t_eval),
+        "latency_ms_per_mesh_end_to_end": (t_build_wall + t_fit + t_fields + t_eval) / a.meshes * 1e3,
         "autotune_table": os.path.basename(a.table) if table else None,
     }
     print(json.dumps(res))
     batch.close()
     for it in items:
         it["g"].close()
+    if nthr > 1:
+        for c in bctx:
+            c.close()
     return res
 
 

@@ -274,7 +274,7 @@ its_batch batch_new(its_ctx ctx) {
 }
 void batch_delete(its_batch b) { delete b; }
 void batch_add(its_batch b, its_svo s) {
-    ITS_REQUIRE(s->ctx == b->ctx, ITS_E_INVALID, "octree belongs to another context");
+    ITS_REQUIRE(s->ctx->device == b->ctx->device, ITS_E_INVALID, "octree is on another device");
     ITS_REQUIRE(s->num_verts > 0, ITS_E_STATE, "empty octree");
     b->svos.push_back(s);
 }
@@ -285,6 +285,17 @@ void batch_add(its_batch b, its_svo s) {
 // solve on one workgroup would bound the launch, run at the same time on the
 // context stream through the multi-workgroup solver.  seconds: wall time of
 // the whole fit.
+// Octrees built on other contexts of the batch's device: their streams
+// finish before the batch's streams read the octrees.
+static void sync_octree_streams(its_batch b) {
+    std::vector<hipStream_t> seen;
+    for (its_svo s : b->svos) {
+        if (s->ctx == b->ctx || std::find(seen.begin(), seen.end(), s->ctx->stream) != seen.end()) continue;
+        seen.push_back(s->ctx->stream);
+        ITS_HIP(hipStreamSynchronize(s->ctx->stream));
+    }
+}
+
 void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_stats* out, double* seconds) {
     const int count = (int)b->svos.size();
     ITS_REQUIRE(count > 0, ITS_E_STATE, "empty batch");
@@ -358,6 +369,7 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
             b->host = h;
         }
         ITS_HIP(hipStreamSynchronize(st));   // inputs written on the context stream are complete
+        sync_octree_streams(b);
         ITS_HIP(hipGraphLaunch(b->gexec, b->side));
     }
     // Large meshes: the multi-workgroup solver, whose inner loop is host
@@ -378,6 +390,7 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
             return b->svos[big[x]]->num_verts > b->svos[big[y]]->num_verts;
         });
         ITS_HIP(hipStreamSynchronize(st));
+        sync_octree_streams(b);
         const int nw = (int)std::min<size_t>(its_batch_s::kWorkers, big.size());
         for (int w = 0; w < nw; ++w)
             if (!b->work[w]) ITS_HIP(hipStreamCreateWithFlags(&b->work[w], hipStreamNonBlocking));
@@ -434,7 +447,7 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
 
 // rebuild the evaluation fields after a batch fit (per mesh)
 void batch_fields(its_batch b) {
-    for (its_svo s : b->svos) field_build(s);
+    for (its_svo s : b->svos) field_build(s);   // each on its octree's stream, which it synchronises
 }
 
 int64_t batch_count(its_batch b) { return (int64_t)b->svos.size(); }

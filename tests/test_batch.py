@@ -93,3 +93,67 @@ def test_batch_warm_and_errors(ctx):
     with pytest.raises(_capi.ItsError):
         batch.eval_device(0, 0, 0, np.zeros(len(items) + 1, np.int64), 0)   # fields not rebuilt yet
     batch.close()
+
+
+def test_batch_from_thread_contexts(ctx):
+    """Octrees built by two host threads, one context each, join a batch on
+    the test's context: fit and evaluation equal those of octrees built on
+    the batch's own context, bit for bit."""
+    import threading
+
+    import torch
+    from its_amd import _capi
+    ctxs = [_capi.Context(0) for _ in range(2)]
+    built, errors = [None] * len(MESHES), []
+
+    def worker(t):
+        try:
+            for m in range(t, len(MESHES), 2):
+                name, R = MESHES[m]
+                c = case(name, R)
+                g = _capi.Octree(ctxs[t], c.tris, c.origin, c.width, R)
+                built[m] = (c, g, g.sdf_at_vertices(c.V, c.F))
+        except Exception as e:   # re-raised below
+            errors.append(e)
+
+    print("[batch] building on two host threads", flush=True)
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(2)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads)
+    if errors:
+        raise errors[0]
+    rng = np.random.default_rng(11)
+    counts = [2000] * len(MESHES)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    Q = np.concatenate([rng.uniform(case(*k).origin - 0.05, case(*k).end + 0.05, size=(n, 3))
+                        for k, n in zip(MESHES, counts)]).astype(np.float32)
+    dq = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
+
+    def fit_eval(items):
+        batch = _capi.Batch(ctx)
+        for _, g, _ in items:
+            batch.add(g)
+        st, _ = batch.fit(tol=1e-6)
+        batch.fields()
+        df = torch.zeros(offs[-1], device="cuda")
+        batch.eval_device(dq[0].data_ptr(), dq[1].data_ptr(), dq[2].data_ptr(), offs, df.data_ptr(), 0.5)
+        lams = [g.get_lambda() for _, g, _ in items]
+        batch.close()
+        return st, lams, df.cpu().numpy()
+
+    st_t, lam_t, f_t = fit_eval(built)
+    st_r, lam_r, f_r = fit_eval(_build(ctx))
+    for a, b in zip(built, _build(ctx)):
+        np.testing.assert_array_equal(a[2], b[2])             # right-hand sides
+    for s1, s2 in zip(st_t, st_r):
+        assert s1["converged"] and s1["iterations"] == s2["iterations"]
+    for a, b in zip(lam_t, lam_r):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(f_t, f_r)
+    for _, g, _ in built:
+        g.close()
+    for c_ in ctxs:
+        c_.close()
