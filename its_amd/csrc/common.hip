@@ -115,5 +115,17 @@ uint64_t scan_exclusive_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStr
 uint64_t rank_build(RankWord* g, int64_t nwords, hipStream_t s) {
     return scan_impl<uint32_t>(PopcLoad{g}, RankStore{g}, RankLoad{g}, nwords, s);
 }
+void rank_build_async(RankWord* g, int64_t nwords, hipStream_t s, uint32_t* d_total) {
+    if (nwords <= 0) {
+        ITS_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
+        return;
+    }
+    // the block-sum buffers are released when this returns: stream-ordered
+    // after the scan when s is the allocation stream, else after a full wait
+    std::vector<DBuf<uint32_t>> keep;
+    keep.reserve(8);
+    const uint32_t* t = scan_launch<uint32_t>(PopcLoad{g}, RankStore{g}, RankLoad{g}, nwords, s, keep);
+    ITS_HIP(hipMemcpyAsync(d_total, t, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+}
 
 }  // namespace its

@@ -456,9 +456,11 @@ void svo_build(its_svo s, const double* d_tris, int64_t T) {
         ITS_LAUNCH_CHECK();
     }
 
-    // ---- vertices per level
+    // ---- vertices per level: every level's lattice bits and ranks are
+    // launched, then all vertex counts come back in one read
     int64_t nv_total = 0;
     std::vector<int64_t> lat_words(depth);
+    DBuf<uint32_t> vcount(depth);
     for (int d = 0; d < depth; ++d) {
         Level& L = s->lv[d];
         const uint32_t D = (uint32_t)L.R + 1;
@@ -473,9 +475,17 @@ void svo_build(its_svo s, const double* d_tris, int64_t T) {
         L.lattice.alloc(nwords);
         hipLaunchKernelGGL(k_pack_bits, dim3(blocks_for(nwords, kT)), dim3(kT), 0, st, lb.p, nwords, L.lattice.p);
         ITS_LAUNCH_CHECK();
-        L.n_verts = (int64_t)rank_build(L.lattice.p, nwords, st);
-        s->esum_verts[d] = nv_total;
-        nv_total += L.n_verts;
+        rank_build_async(L.lattice.p, nwords, st, vcount.p + d);
+    }
+    {
+        std::vector<uint32_t> h(depth);
+        ITS_HIP(hipMemcpyAsync(h.data(), vcount.p, depth * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipStreamSynchronize(st));
+        for (int d = 0; d < depth; ++d) {
+            s->lv[d].n_verts = (int64_t)h[d];
+            s->esum_verts[d] = nv_total;
+            nv_total += s->lv[d].n_verts;
+        }
     }
     s->esum_verts[depth] = nv_total;
     s->num_verts = nv_total;
