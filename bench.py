@@ -215,9 +215,10 @@ def main():
     pinned_ok = bool(torch.equal(fh, out.cpu()))
     q_aos = qh.numpy().T.copy()
     t_host = wall(lambda: svo.eval(q_aos, end))
+    host_ok = bool(np.array_equal(svo.eval(q_aos, end), out.cpu().numpy()))
     e2e = {"pinned_soa_ms": t_pinned * 1e3, "pinned_evals_per_s": N_QUERIES / t_pinned,
            "host_entry_ms": t_host * 1e3, "host_entry_evals_per_s": N_QUERIES / t_host,
-           "pinned_equals_resident": pinned_ok}
+           "pinned_equals_resident": pinned_ok, "host_entry_equals_resident": host_ok}
     log(f"[bench] end to end: pinned {t_pinned * 1e3:.2f} ms, host entry point {t_host * 1e3:.2f} ms")
     del qh, fh, qd, q_aos
 

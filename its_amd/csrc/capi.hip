@@ -498,14 +498,10 @@ int its_bspline_eval(its_ctx ctx, its_svo s, const float* pts, int64_t N, const 
     AllocStream as(ctx->stream);
     hipStream_t st = ctx->stream;
     if (N == 0) return ITS_OK;
-    std::vector<float> soa(3 * (size_t)N);
-    for (int64_t i = 0; i < N; ++i) {
-        soa[i] = pts[3 * i];
-        soa[N + i] = pts[3 * i + 1];
-        soa[2 * N + i] = pts[3 * i + 2];
-    }
-    DBuf<float> d = upload(soa.data(), soa.size(), st), out(N);
-    eval_points(s, d.p, d.p + N, d.p + 2 * N, N, box_end, mode, out_val, out.p, st);
+    // the caller's interleaved points go to the device as they are; the
+    // kernel reads them interleaved (no reordering pass on the host)
+    DBuf<float> d = upload(pts, 3 * (size_t)N, st), out(N);
+    eval_points_aos32(s, d.p, N, box_end, mode, out_val, out.p, st);
     ITS_HIP(hipMemcpyAsync(f, out.p, N * sizeof(float), hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
     ITS_API_END

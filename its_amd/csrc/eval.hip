@@ -106,14 +106,7 @@ __device__ inline bool box64(const EvalParams& P, double dx, double dy, double d
     return true;
 }
 
-struct SoaF32 {
-    const float *x, *y, *z;
-    __device__ void operator()(int64_t i, float& a, float& b, float& c, double& da, double& db, double& dc) const {
-        a = x[i];
-        b = y[i];
-        c = z[i];
-        da = db = dc = 0.0;
-    }
+struct F32Box {
     // For binary32 inputs the binary64 comparisons are exact in binary32
     // against bounds rounded outward on the host (flo/fhi).
     __device__ bool in_box(const EvalParams& P, float x, float y, float z, double, double, double) const {
@@ -126,6 +119,27 @@ struct SoaF32 {
             return !((x < P.flo[0]) | (y < P.flo[1]) | (z < P.flo[2]) | (x > P.fhi[0]) | (y > P.fhi[1]) |
                      (z > P.fhi[2]));
         return true;
+    }
+};
+struct SoaF32 : F32Box {
+    const float *x, *y, *z;
+    __host__ __device__ SoaF32(const float* x_, const float* y_, const float* z_) : x(x_), y(y_), z(z_) {}
+    __device__ void operator()(int64_t i, float& a, float& b, float& c, double& da, double& db, double& dc) const {
+        a = x[i];
+        b = y[i];
+        c = z[i];
+        da = db = dc = 0.0;
+    }
+};
+// interleaved x, y, z per query: the layout of a caller's point array
+struct AosF32 : F32Box {
+    const float* p;
+    __host__ __device__ explicit AosF32(const float* p_) : p(p_) {}
+    __device__ void operator()(int64_t i, float& a, float& b, float& c, double& da, double& db, double& dc) const {
+        a = p[3 * i];
+        b = p[3 * i + 1];
+        c = p[3 * i + 2];
+        da = db = dc = 0.0;
     }
 };
 struct AosF64 {
@@ -915,6 +929,13 @@ void eval_points(its_svo s, const float* x, const float* y, const float* z, int6
     EvalParams P = make_params(s, box_end, mode, out_val);
     if (N <= 0) return;
     launch_eval(SoaF32{x, y, z}, s, N, P, f, st);
+}
+
+void eval_points_aos32(its_svo s, const float* pts, int64_t N, const double box_end[3], int mode, float out_val,
+                       float* f, hipStream_t st) {
+    EvalParams P = make_params(s, box_end, mode, out_val);
+    if (N <= 0) return;
+    launch_eval(AosF32{pts}, s, N, P, f, st);
 }
 
 void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box_end[3], int mode, float out_val,

@@ -106,6 +106,8 @@ void dense_cascade(its_svo s, int stop, DBuf<float>& out);
 void dense_cascade64(its_svo s, int stop, DBuf<double>& out);
 void eval_points(its_svo s, const float* x, const float* y, const float* z, int64_t N, const double box_end[3],
                  int mode, float out_val, float* f, hipStream_t st);
+void eval_points_aos32(its_svo s, const float* pts, int64_t N, const double box_end[3], int mode, float out_val,
+                       float* f, hipStream_t st);
 void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box_end[3], int mode, float out_val,
                        float* f, hipStream_t st);
 // lscg.hip
