@@ -29,6 +29,8 @@ void eval_lattice(its_svo s, const double o[3], const double vs[3], const int64_
 
 namespace {
 constexpr int kT = 256;
+constexpr int kStepBits = 28;   // steps per grid < 2^28
+constexpr unsigned long long kStepMask = (1ull << kStepBits) - 1;
 __constant__ __align__(16) signed char c_tri[256][16];
 __constant__ unsigned char c_nv[256];
 
@@ -62,9 +64,14 @@ __device__ inline unsigned case8(const float (&v)[8], float iso) {
     return c;
 }
 
-// pass 1: cube cases, vertices per step, voxels with triangles
+// pass 1: cube cases, vertices per step, voxels with triangles, and the
+// list of steps with vertices.  One 64-bit counter holds both totals: listed
+// steps in the low kStepBits bits, voxels with triangles above them, so a
+// wave makes one atomic.  The list order depends on timing; each step's place
+// in the soup comes from the scan of step_verts, so the output does not.
 __global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, McGrid G, uint8_t* __restrict__ cases,
-                                                 uint64_t* __restrict__ step_verts, unsigned long long* nvalid) {
+                                                 uint64_t* __restrict__ step_verts, unsigned long long* counts,
+                                                 uint32_t* __restrict__ steps) {
     const int lane = threadIdx.x & 63;
     const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
     if (w >= G.nwaves) return;   // whole waves only
@@ -82,6 +89,7 @@ __global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, 
     float na0 = lat[p0 + (j0 + 1) * G.lx], na1 = lat[p0 + (j0 + 1) * G.lx + 1];
     float nb0 = lat[p1 + (j0 + 1) * G.lx], nb1 = lat[p1 + (j0 + 1) * G.lx + 1];
     unsigned nval = 0;
+    uint32_t mask = 0;   // bit j - j0: step (k, j, ch) has vertices (J <= 32)
     for (int j = j0; j < j1; ++j) {
         const int64_t jn = min(j + 2, G.ry) * G.lx;
         const float f0 = lat[p0 + jn], f1 = lat[p0 + jn + 1], g0 = lat[p1 + jn], g1 = lat[p1 + jn + 1];
@@ -93,10 +101,18 @@ __global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, 
         const unsigned sum = wave_sum(nv);
         nval += __popcll(__ballot(nv != 0));
         if (lane == 0) step_verts[((int64_t)k * G.ry + j) * G.nch + ch] = sum;
+        if (sum) mask |= 1u << (j - j0);
         a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
         na0 = f0; na1 = f1; nb0 = g0; nb1 = g1;
     }
-    if (lane == 0 && nval) atomicAdd(nvalid, (unsigned long long)nval);
+    if (!mask) return;
+    unsigned long long slot = 0;
+    if (lane == 0)
+        slot = atomicAdd(counts, ((unsigned long long)nval << kStepBits) | (unsigned long long)__popc(mask)) &
+               kStepMask;
+    slot = __shfl(slot, 0, 64);
+    if (lane < 32 && ((mask >> lane) & 1u))
+        steps[slot + __popc(mask & ((1u << lane) - 1u))] = (uint32_t)(((int64_t)k * G.ry + j0 + lane) * G.nch + ch);
 }
 
 struct P3 {
@@ -129,18 +145,13 @@ __device__ inline P3 corner_pos(const P3& vp, double vx, double vy, double vz, i
               vp.z + (((kCornerZ >> q) & 1) ? vz : 0.0)};
 }
 
-// pass 2: one wave per step; a step without vertices ends after one load.  A
-// wave prefix sum over its 64 voxels places their triangles after those of
-// all earlier voxels (step_off: exclusive scan of the step counts).
-__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
-                                                double oz, double vx, double vy, double vz,
-                                                const uint8_t* __restrict__ cases, const uint64_t* __restrict__ step_off,
-                                                uint64_t total, uint64_t max_verts, float* __restrict__ out) {
+// pass 2: one wave per listed step (64 voxels of a row).  A wave prefix sum
+// over the 64 voxels places their triangles after those of all earlier
+// voxels (step_off: exclusive scan of the step counts).
+__device__ inline void emit_step(const float* __restrict__ lat, const McGrid& G, double ox, double oy, double oz,
+                                 double vx, double vy, double vz, const uint8_t* __restrict__ cases, uint64_t at,
+                                 int64_t s, uint64_t max_verts, float* __restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const int64_t s = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
-    if (s >= G.nsteps) return;
-    const uint64_t at = step_off[s];
-    if ((s + 1 < G.nsteps ? step_off[s + 1] : total) == at) return;   // no vertices in this step
     const int ch = (int)(s % G.nch);
     const int64_t row = s / G.nch;
     const int j = (int)(row % G.ry), k = (int)(row / G.ry);
@@ -151,7 +162,7 @@ __global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, M
         nv = c_nv[c];
     }
     const uint32_t incl = wave_incl_scan(nv);
-    if (!nv) return;
+    if (!nv) return;   // after the wave scan: lanes part here
     const int64_t base = ((int64_t)k * G.ly + j) * G.lx;
     // voxelToMeshKernel: voxelPos = shift * voxelSize + origin
     const P3 vp{(double)i * vx + ox, (double)j * vy + oy, (double)k * vz + oz};
@@ -176,6 +187,19 @@ __global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, M
             out[3 * (pos + q) + 2] = (float)p.z;
         }
     }
+}
+
+__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
+                                                double oz, double vx, double vy, double vz,
+                                                const uint8_t* __restrict__ cases, const uint64_t* __restrict__ step_off,
+                                                const uint32_t* __restrict__ steps,
+                                                const unsigned long long* __restrict__ counts, uint64_t max_verts,
+                                                float* __restrict__ out) {
+    const int64_t nlist = (int64_t)min(counts[0] & kStepMask, (unsigned long long)G.nsteps);
+    const int64_t e = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+    if (e >= nlist) return;   // whole waves
+    const uint32_t s = steps[e];
+    emit_step(lat, G, ox, oy, oz, vx, vy, vz, cases, step_off[s], (int64_t)s, max_verts, out);
 }
 
 // once per device, also when several host threads make their first call
@@ -286,31 +310,35 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     G.iso = (float)iso;
     s->mc_case.ensure(nvox);
     s->mc_step.ensure(G.nsteps);
+    ITS_REQUIRE(G.nsteps < (int64_t)kStepMask, ITS_E_RANGE, "marching-cubes grid has too many rows");
+    s->mc_list.ensure(G.nsteps);
     s->mc_valid.ensure(1);
     ITS_HIP(hipMemsetAsync(s->mc_valid.p, 0, sizeof(unsigned long long), st));
     hipLaunchKernelGGL(k_mc_count, dim3((unsigned)((G.nwaves * 64 + kT - 1) / kT)), dim3(kT), 0, st,
-                       (const float*)s->mc_lat.p, G, s->mc_case.p, s->mc_step.p, s->mc_valid.p);
+                       (const float*)s->mc_lat.p, G, s->mc_case.p, s->mc_step.p, s->mc_valid.p, s->mc_list.p);
     ITS_LAUNCH_CHECK();
+    // the packed counter comes back with the scan's total (the scan synchronises)
+    unsigned long long counts = 0;
+    ITS_HIP(hipMemcpyAsync(&counts, s->mc_valid.p, sizeof counts, hipMemcpyDeviceToHost, st));
     const uint64_t total = scan_exclusive_u64(s->mc_step.p, s->mc_step.p, G.nsteps, st);
+    const int64_t nlist = (int64_t)(counts & kStepMask);
     S.num_verts = (int64_t)total;
     const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
     if (h_tri && total) {
         ITS_REQUIRE((int64_t)total <= capacity, ITS_E_RANGE, "triangle buffer too small");
         s->mc_soup.ensure(3 * total);
-        hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((G.nsteps * 64 + kT - 1) / kT)), dim3(kT), 0, st,
+        hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((nlist * 64 + kT - 1) / kT)), dim3(kT), 0, st,
                            (const float*)s->mc_lat.p, G, origin[0], origin[1], origin[2], vs[0], vs[1], vs[2],
-                           (const uint8_t*)s->mc_case.p, (const uint64_t*)s->mc_step.p, total, max_verts,
-                           s->mc_soup.p);
+                           (const uint8_t*)s->mc_case.p, (const uint64_t*)s->mc_step.p, (const uint32_t*)s->mc_list.p,
+                           (const unsigned long long*)s->mc_valid.p, max_verts, s->mc_soup.p);
         ITS_LAUNCH_CHECK();
         ITS_HIP(hipEventRecord(ev[2], st));
         ITS_HIP(hipMemcpyAsync(h_tri, s->mc_soup.p, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
     } else {
         ITS_HIP(hipEventRecord(ev[2], st));
     }
-    unsigned long long nvalid = 0;
-    ITS_HIP(hipMemcpyAsync(&nvalid, s->mc_valid.p, sizeof nvalid, hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
-    S.num_valid_voxels = (int64_t)nvalid;
+    S.num_valid_voxels = (int64_t)(counts >> kStepBits);
     auto t2 = std::chrono::steady_clock::now();
     float ms_lat = 0.f, ms_ext = 0.f;
     ITS_HIP(hipEventElapsedTime(&ms_lat, ev[0], ev[1]));
