@@ -302,6 +302,16 @@ int its_tune_get(int key, int64_t* value);
 int its_timer_start(its_ctx ctx);
 int its_timer_stop(its_ctx ctx, double* ms);
 
+/* ---- achievable HBM bandwidth of the context's device ----
+ * Copies `bytes` (rounded down to a multiple of 16; at least 1 MiB, at most
+ * 16 GiB) device to device with the library's own streaming kernel: two
+ * untimed launches, then `reps` (1..100) launches timed one by one with
+ * events on the context stream.  *gbps = 2 * bytes / median launch time
+ * (bytes read plus bytes written), in 1e9 B/s.  The call allocates and frees
+ * its two buffers.  Kernels are reported against this as well as against the
+ * published peak.  No reference counterpart. */
+int its_hbm_bandwidth(its_ctx ctx, int64_t bytes, int reps, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,7 @@ SIGNATURES = {
     "its_warm_apply": [vp, vp, _f64p, _f64p],
     "its_lscg_solve_mlp": [vp, vp, vp, vp, _f64p, C.c_double, C.c_int64, C.POINTER(LscgStats),
                            C.POINTER(C.c_double)],
+    "its_hbm_bandwidth": [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)],
 }
 _RET = {"its_last_error": C.c_char_p, "its_version": C.c_char_p}
 
@@ -227,6 +228,13 @@ class Context:
         ms = C.c_double(0)
         check(lib().its_timer_stop(self._h, C.byref(ms)))
         return ms.value
+
+    def hbm_bandwidth(self, nbytes=1 << 30, reps=10):
+        """Achievable device bandwidth in GB/s (read + write) of a plain
+        streaming copy of nbytes, median of reps launches."""
+        g = C.c_double(0)
+        check(lib().its_hbm_bandwidth(self._h, int(nbytes), int(reps), C.byref(g)))
+        return g.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -193,3 +193,17 @@ def test_eval_boundary_points(ctx, name, R):
     inside = np.all((Pd >= c.origin - c.width) & (Pd <= c.end + c.width), axis=1)
     assert np.all(fd[~inside] == np.float32(7.0)) and np.all(fd[inside] != np.float32(7.0))
     g.close()
+
+
+def test_hbm_bandwidth_probe(ctx):
+    """The streaming-copy probe that the roofline report divides by: a
+    positive rate below the published 8 TB/s, and range errors for sizes and
+    repetition counts outside the documented bounds."""
+    from its_amd import _capi
+    print("[eval] streaming copy of 256 MiB", flush=True)
+    g = ctx.hbm_bandwidth(256 << 20, 5)
+    assert 100.0 < g < 8000.0, g
+    for nbytes, reps in [(1 << 10, 5), (256 << 20, 0), (256 << 20, 101)]:
+        with pytest.raises(_capi.ItsError) as e:
+            ctx.hbm_bandwidth(nbytes, reps)
+        assert e.value.code == -5
