@@ -88,7 +88,7 @@ struct its_svo_s {
     its::DBuf<float> mc_lat, mc_soup;
     its::DBuf<uint8_t> mc_case;
     its::DBuf<uint64_t> mc_step;
-    its::DBuf<unsigned long long> mc_valid;   // voxels with triangles and listed steps, packed
+    its::DBuf<unsigned long long> mc_valid;   // [0] voxels with triangles and listed steps, packed; [1] vertices
     its::DBuf<uint32_t> mc_list;              // steps with vertices
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};

@@ -312,15 +312,19 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     s->mc_step.ensure(G.nsteps);
     ITS_REQUIRE(G.nsteps < (int64_t)kStepMask, ITS_E_RANGE, "marching-cubes grid has too many rows");
     s->mc_list.ensure(G.nsteps);
-    s->mc_valid.ensure(1);
+    s->mc_valid.ensure(2);
     ITS_HIP(hipMemsetAsync(s->mc_valid.p, 0, sizeof(unsigned long long), st));
     hipLaunchKernelGGL(k_mc_count, dim3((unsigned)((G.nwaves * 64 + kT - 1) / kT)), dim3(kT), 0, st,
                        (const float*)s->mc_lat.p, G, s->mc_case.p, s->mc_step.p, s->mc_valid.p, s->mc_list.p);
     ITS_LAUNCH_CHECK();
-    // the packed counter comes back with the scan's total (the scan synchronises)
-    unsigned long long counts = 0;
-    ITS_HIP(hipMemcpyAsync(&counts, s->mc_valid.p, sizeof counts, hipMemcpyDeviceToHost, st));
-    const uint64_t total = scan_exclusive_u64(s->mc_step.p, s->mc_step.p, G.nsteps, st);
+    // the packed counter and the scan's total come back in one read
+    static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "counter width");
+    scan_exclusive_u64_async(s->mc_step.p, s->mc_step.p, G.nsteps, st, (uint64_t*)s->mc_valid.p + 1);
+    unsigned long long back[2];
+    ITS_HIP(hipMemcpyAsync(back, s->mc_valid.p, sizeof back, hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    const unsigned long long counts = back[0];
+    const uint64_t total = back[1];
     const int64_t nlist = (int64_t)(counts & kStepMask);
     S.num_verts = (int64_t)total;
     const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
