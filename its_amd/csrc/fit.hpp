@@ -10,9 +10,10 @@ struct Coarse {
 };
 
 // Operator gathers shared by the single-mesh and the batched solver.  Index
-// lists are read 8 entries at a time and then the 8 values, so a thread
-// waits for two loads per 8 entries instead of two per entry; the sums keep
-// the list order.
+// lists are read a chunk at a time and then the chunk's values, so a thread
+// waits for two loads per chunk instead of two per entry; the sums keep the
+// list order.  Parent lists hold at most 8 entries (one chunk of 8); child
+// lists up to 27 (one chunk of 32).
 // sum of g over the prolongation list [k0, k1) (coarse parents of a vertex)
 template <class T>
 __device__ inline T gather_parents(const uint32_t* idx, uint32_t k0, uint32_t k1, const T* g) {
@@ -34,16 +35,17 @@ __device__ inline T gather_parents(const uint32_t* idx, uint32_t k0, uint32_t k1
 // log2 of the child's parent count k in bits 28 and up), weights exact
 template <class T>
 __device__ inline T gather_children(const uint32_t* ridx, uint32_t k0, uint32_t k1, const T* h) {
+    constexpr int W = 32;
     T s = 0;
-    for (uint32_t kb = k0; kb < k1; kb += 8) {
-        uint32_t v[8];
-        T hv[8];
+    for (uint32_t kb = k0; kb < k1; kb += W) {
+        uint32_t v[W];
+        T hv[W];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : 0u;
+        for (int q = 0; q < W; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : 0u;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) hv[q] = (kb + q < k1) ? h[v[q] & 0x0FFFFFFFu] : (T)0;
+        for (int q = 0; q < W; ++q) hv[q] = (kb + q < k1) ? h[v[q] & 0x0FFFFFFFu] : (T)0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+        for (int q = 0; q < W; ++q)
             if (kb + q < k1) s += ldexp(hv[q], -(int)(v[q] >> 28));
     }
     return s;

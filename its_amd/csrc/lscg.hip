@@ -340,18 +340,18 @@ __global__ __launch_bounds__(kCoarseT) void k_restrict_coarse(Coarse C, const ui
         for (uint32_t c = C.off[d] + threadIdx.x; c < C.off[d + 1]; c += blockDim.x) {
             float s = 0.f;
             const uint32_t k0 = ptr[c], k1 = ptr[c + 1];
-            for (uint32_t kb = k0; kb < k1; kb += 8) {
-                uint32_t v[8];
-                float hy[8];
+            for (uint32_t kb = k0; kb < k1; kb += 32) {   // child lists: at most 27 entries
+                uint32_t v[32];
+                float hy[32];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : base;   // base: in bounds for both reads
+                for (int q = 0; q < 32; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : base;   // base: in bounds for both reads
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
+                for (int q = 0; q < 32; ++q) {
                     const uint32_t y = v[q] & 0x0FFFFFFFu;
                     hy[q] = (kb + q < k1) ? ((d == C.lc) ? h[y] : sh[y - base]) : 0.f;
                 }
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
+                for (int q = 0; q < 32; ++q)
                     if (kb + q < k1) s += ldexpf(hy[q], -(int)(v[q] >> 28));
             }
             float hc = h[c] + s;
