@@ -9,9 +9,9 @@
 // evaluated once per lattice point ((res+1)^3 points), by the dense cascade on
 // the octree's own lattice or by the octree field walk elsewhere.  A count
 // pass writes every voxel's cube case and the vertex count of every step of
-// 64 voxels along x; an exclusive scan of the step counts gives each step its
-// offset; an emit pass, one wave per step, places each voxel's triangles with
-// a wave prefix sum, and steps without triangles end after one load.  The
+// 64 voxels along x, and lists the steps that have vertices; an exclusive
+// scan of the step counts gives each step its offset; an emit pass, one wave
+// per listed step, places each voxel's triangles with a wave prefix sum.  The
 // soup order is the reference's: voxels x-fastest, triangles in table order.
 // The 1e-6 snaps of vertexLerp (MarchingCubes.cu:72-90) are kept, and so is
 // the lerp fma(t, v1, fma(-t, v0, v0)).
