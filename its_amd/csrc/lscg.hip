@@ -302,18 +302,31 @@ __global__ __launch_bounds__(kT) void k_restrict(int64_t b, int64_t e, const uin
 // between levels, replacing one launch per level.
 constexpr int kCoarse = 12288;
 constexpr int kCoarseT = 1024;
+// The list offsets of the coarse range are staged in LDS first (one round of
+// loads) when the range holds fewer than kCoarsePtr vertices, so a level
+// waits for its list entries and values only.
+constexpr int kCoarsePtr = 2049;
+__device__ inline bool stage_ptr(const Coarse& C, const uint32_t* ptr, uint32_t* sptr) {
+    const uint32_t base = C.off[C.lc], nptr = C.off[C.L] - base + 1;
+    if (nptr > (uint32_t)kCoarsePtr) return false;
+    for (uint32_t t = threadIdx.x; t < nptr; t += blockDim.x) sptr[t] = ptr[base + t];
+    __syncthreads();
+    return true;
+}
 __global__ __launch_bounds__(kCoarseT) void k_prolong_coarse(Coarse C, const uint32_t* ptr, const uint32_t* idx,
                                                             float* p, const float* z, float* g, const double* scal,
                                                             const int* ctl, int upd) {
     if (upd && ctl[0]) return;
     __shared__ float sg[kCoarse];
+    __shared__ uint32_t sptr[kCoarsePtr];
     const uint32_t base = C.off[C.lc];
+    const bool staged = stage_ptr(C, ptr, sptr);
     const float beta = upd ? (float)scal[S_BETA] : 0.f;
     for (int d = C.L - 1; d >= C.lc; --d) {
         for (uint32_t i = C.off[d] + threadIdx.x; i < C.off[d + 1]; i += blockDim.x) {
             float pi = p[i];
             if (upd) { pi = z[i] + beta * pi; p[i] = pi; }
-            uint32_t k0 = ptr[i], k1 = ptr[i + 1];
+            const uint32_t k0 = staged ? sptr[i - base] : ptr[i], k1 = staged ? sptr[i + 1 - base] : ptr[i + 1];
             float s = 0.f;
             for (uint32_t kb = k0; kb < k1; kb += 8) {
                 uint32_t id[8];
@@ -335,11 +348,13 @@ __global__ __launch_bounds__(kCoarseT) void k_restrict_coarse(Coarse C, const ui
                                                              float* h, const int* ctl, int check) {
     if (check && ctl[0]) return;
     __shared__ float sh[kCoarse];
+    __shared__ uint32_t sptr[kCoarsePtr];
     const uint32_t base = C.off[C.lc];
+    const bool staged = stage_ptr(C, ptr, sptr);
     for (int d = C.lc; d < C.L; ++d) {
         for (uint32_t c = C.off[d] + threadIdx.x; c < C.off[d + 1]; c += blockDim.x) {
             float s = 0.f;
-            const uint32_t k0 = ptr[c], k1 = ptr[c + 1];
+            const uint32_t k0 = staged ? sptr[c - base] : ptr[c], k1 = staged ? sptr[c + 1 - base] : ptr[c + 1];
             for (uint32_t kb = k0; kb < k1; kb += 32) {   // child lists: at most 27 entries
                 uint32_t v[32];
                 float hy[32];
