@@ -47,13 +47,25 @@ def make_mesh():
     return V, F, origin, end, width
 
 
-def cpu_baseline(V, F, origin, end, width, lam, q_host, budget_s=15.0):
-    """Oracle (FP64 CPU restatement of mt_cpuTest_2) on a prefix of the queries."""
+def cpu_baseline(V, F, origin, end, width, lam, q_host, f_gpu, gpu_svo, budget_s=15.0):
+    """Oracle (FP64 CPU restatement of mt_cpuTest_2) on a prefix of the
+    queries, timed; its values double as the parity check of the timed GPU
+    output, and the oracle's octree is compared with the GPU's bit for bit."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from its_amd import meshes
 
     svo = O.SVO(meshes.triangles(V, F), origin, width, R)
+    nd = gpu_svo.nodes()
+    xyz, owner = gpu_svo.verts()
+    bit_exact = bool(svo.num_nodes == gpu_svo.num_nodes and svo.num_verts == gpu_svo.num_verts
+                     and all(np.array_equal(nd[k], r) for k, r in
+                             [("morton", svo.morton), ("is_leaf", svo.is_leaf), ("parent", svo.parent),
+                              ("childs", svo.childs), ("neighbors", svo.neighbors)])
+                     and np.array_equal(nd["origin"].view(np.uint64), svo.node_origin.view(np.uint64))
+                     and np.array_equal(nd["width"].view(np.uint64), svo.node_width.view(np.uint64))
+                     and np.array_equal(owner, svo.vert_owner)
+                     and np.array_equal(xyz.view(np.uint64), svo.vert_xyz.view(np.uint64)))
     threads = O.lib().ora_num_threads()
     n = 20000
     t0 = time.perf_counter()
@@ -61,15 +73,23 @@ def cpu_baseline(V, F, origin, end, width, lam, q_host, budget_s=15.0):
     dt = time.perf_counter() - t0
     n2 = int(min(len(q_host), max(n, budget_s / max(dt, 1e-9) * n)))
     t0 = time.perf_counter()
-    svo.eval_support(lam, q_host[:n2].astype(np.float64), end, 0.0)
+    f_ref = svo.eval_support(lam, q_host[:n2].astype(np.float64), end, 0.0)
     dt2 = time.perf_counter() - t0
-    return {
+    d = np.abs(f_gpu[:n2].astype(np.float64) - f_ref)
+    parity = {"svo_bit_exact": bit_exact, "nodes": int(svo.num_nodes), "verts": int(svo.num_verts),
+              "f_queries_checked": n2, "f_max_abs": float(d.max()),
+              "f_max_rel": float((d / np.maximum(np.abs(f_ref), 1e-3)).max()), "f_tol": 1e-5,
+              "f_within_tol": bool(d.max() <= 1e-5),
+              "note": "timed GPU output vs the FP64 oracle support evaluation with the same coefficients; "
+                      "relative error against max(|f|, 1e-3)"}
+    cpu = {
         "value": n2 / dt2,
         "unit": "evals/s",
         "cores": threads,
         "kind": "port",
         "sample": f"first {n2} of the {N_QUERIES} queries, oracle support evaluation (FP64, OpenMP)",
     }
+    return cpu, parity
 
 
 def main():
@@ -135,6 +155,23 @@ def main():
     t_lscg = float(np.median([s["seconds"] for s in lscg_runs]))
     st = lscg_runs[-1]
     log(f"[bench] lscg cold: {t_lscg*1e3:.2f} ms, iters={st['iterations']} normal_rel={st['normal_rel']:.2e}")
+    # LSCG roofline: bytes of one CGLS iteration over the time of one
+    # iteration (the whole solve, set-up and binary64 refinement included,
+    # divided by the iteration count): the matrix-free operator's own count,
+    # and the assembled matrix's 16 B per non-zero + 72 B per row
+    ls = svo.lscg_structure()
+    t_it = t_lscg / max(st["iterations"], 1)
+    lscg_roof = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "iterations": st["iterations"],
+                 "seconds_per_iteration": t_it, "nnz": ls["nnz"], "n": ls["n"],
+                 "parent_entries": ls["parent_entries"],
+                 "bytes_matrix_free_per_iteration": ls["bytes_matrix_free_per_iteration"],
+                 "achieved": ls["bytes_matrix_free_per_iteration"] / t_it / 1e9,
+                 "frac": ls["bytes_matrix_free_per_iteration"] / t_it / 1e9 / HBM_PEAK_GBS,
+                 "bytes_assembled_per_iteration": ls["bytes_assembled_per_iteration"],
+                 "assembled_equivalent": ls["bytes_assembled_per_iteration"] / t_it / 1e9,
+                 "frac_assembled_equivalent": ls["bytes_assembled_per_iteration"] / t_it / 1e9 / HBM_PEAK_GBS}
+    log(f"[bench] lscg roofline: {t_it * 1e6:.1f} us per iteration, matrix-free "
+        f"{lscg_roof['achieved']:.0f} GB/s, assembled-equivalent {lscg_roof['assembled_equivalent']:.0f} GB/s")
 
     # marching cubes on the 256^3 grid of the octree box
     # the coefficients are set again before each run, so each run evaluates
@@ -267,10 +304,12 @@ def main():
         log(f"[bench] warm start: {warm['mean_iters_cold']:.1f} -> {warm['mean_iters_warm']:.1f} iterations, "
             f"speedup {warm['speedup']:.2f}")
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
-        log("[bench] cpu baseline")
-        cpu = cpu_baseline(V, F, origin, end, width, lam, q.cpu().numpy().T.copy())
+        log("[bench] cpu baseline and parity")
+        cpu, parity = cpu_baseline(V, F, origin, end, width, lam, q.cpu().numpy().T.copy(), out.cpu().numpy(), svo)
+        log(f"[bench] parity: octree bit-exact {parity['svo_bit_exact']}, f max abs {parity['f_max_abs']:.2e} "
+            f"over {parity['f_queries_checked']} queries")
 
     res = {
         "metric": "B-spline evals/s @depth 8; LSCG time-to-1e-6 residual, warm vs cold start",
@@ -297,7 +336,7 @@ def main():
         "eval_kernel_ms": {"mean": avg_ms},
         "eval_end_to_end": e2e,
         "lscg_cold": {"seconds": t_lscg, "iterations": st["iterations"], "normal_rel": st["normal_rel"],
-                      "residual_rel": st["residual_rel"]},
+                      "residual_rel": st["residual_rel"], "roofline": lscg_roof},
         "lscg_warm": warm,
         "mc": mc,
         "setup_seconds": {"svo_build": t_svo_warm, "sdf_rhs": t_sdf_warm, "svo_build_first_call": t_svo,
@@ -306,6 +345,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "measured_peak": stream_gbs,
                      "frac_of_measured": achieved / stream_gbs},
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(res))
 

@@ -66,6 +66,20 @@ int its_svo_build(its_ctx ctx, const double* tris, int64_t T, const double origi
 int its_svo_build_device(its_ctx ctx, const double* d_tris, int64_t T, const double origin[3], const double width[3],
                          int R, its_svo* out);
 int its_svo_destroy(its_svo svo);
+/* An octree object that holds only a vertex set, from the arrays the
+ * reference passes to MC::marching_cubes (MarchingCubes.h:136-139):
+ * nodeVertexArray as xyz (n_verts x 3) and owner node (n_verts), and
+ * nodeWidthArray (n_nodes x 3, the x component is used).  A vertex's level
+ * follows from its owner's width (the smallest width times 2^level), the grid
+ * origin is the smallest coordinate of each axis (the root's first corner).
+ * Such an octree supports its_svo_get_info, its_svo_export_verts and
+ * its_svo_export_vert_lattice (level / lattice order), coefficients
+ * (its_svo_set_lambda / get_lambda, in the caller's vertex order),
+ * evaluation, marching cubes and the lattice entry points; the others
+ * return ITS_E_STATE.  Every vertex must be a corner of its level's lattice,
+ * and no two vertices may share a level and a lattice point (ITS_E_INVALID). */
+int its_svo_from_vertices(its_ctx ctx, const double* xyz, const uint32_t* owner, int64_t n_verts,
+                          const double* node_width, int64_t n_nodes, its_svo* out);
 
 typedef struct {
     int tree_depth;                               /* treeDepth (levels), finest = 0 */
@@ -133,6 +147,22 @@ int its_lscg_solve(its_ctx ctx, its_svo svo, const double* b, double* lambda_ino
 /* Matrix-free products with the collocation matrix A_ij = B_j(v_i)
  * (the assembled matrix of ThinShells.cpp:556-605), binary64, host vectors. */
 int its_collocation_mul(its_ctx ctx, its_svo svo, const double* x, double* y, int transpose);
+/* Sizes of the operator: n, the non-zeros of the matrix the reference
+ * assembles (ThinShells.cpp:556-605, counted on the device), the entries of
+ * the matrix-free operator's parent lists, the levels (and how many of them
+ * the one-workgroup kernels take), and the bytes one CGLS iteration moves:
+ * 16 B per non-zero + 72 B per row for the assembled matrix, and this
+ * library's matrix-free count (89 B per vertex + 16 B per parent entry). */
+typedef struct {
+    int64_t n;
+    int64_t nnz;
+    int64_t parent_entries;
+    int levels;
+    int block_levels;
+    double bytes_assembled_per_iteration;
+    double bytes_matrix_free_per_iteration;
+} its_lscg_structure;
+int its_lscg_structure_get(its_ctx ctx, its_svo svo, its_lscg_structure* out);
 /* The solver's Jacobi diagonal 1/||A_:j||^2 (1 for an empty column), the
  * LeastSquareDiagonalPreconditioner of the Eigen solve; host n-vector. */
 int its_lscg_diag(its_ctx ctx, its_svo svo, double* invd);
@@ -178,13 +208,19 @@ int its_shell_iso(its_ctx ctx, its_svo svo, const double* V, int64_t nV, const i
 int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, int8_t* cls);
 
 /* ---- marching cubes (MC::marching_cubes, MarchingCubes.h:136-139,
- *      MarchingCubes.cu:672-714) ----
- * Grid of res[0] x res[1] x res[2] voxels over origin + [0, width).  The
- * triangle soup is returned as float xyz, 3 vertices per triangle, in the
- * reference order (voxels x-fastest, triangles in table order).
- * tri_xyz may be NULL to query the count; capacity is in vertices.  The
- * octree keeps the last lattice, so a fill call after its count call, or an
- * extraction at another iso value on the same grid, evaluates no lattice. */
+ *      MarchingCubes.cu:672-714; the three iso values of mcVisualization,
+ *      ThinShells.cpp:1086-1129) ----
+ * Grid of res[0] x res[1] x res[2] voxels over origin + [0, width).
+ * lambda: new coefficients (host n-vector, as its_svo_set_lambda) or NULL
+ * for the octree's current ones.  iso: n_iso values (1..ITS_MC_MAX_ISO); one
+ * soup per value, concatenated in that order, each as float xyz, 3 vertices
+ * per triangle, in the reference order (voxels x-fastest, triangles in table
+ * order).  num_verts (may be NULL): n_iso counts.  tri_xyz may be NULL to
+ * query the counts; capacity is in vertices (all soups together).  stats:
+ * sums over the iso values.  The octree keeps the last lattice, so a fill
+ * call after its count call, and every iso value after the first on one
+ * grid, evaluate no lattice. */
+#define ITS_MC_MAX_ISO 16
 typedef struct {
     int64_t num_verts;        /* allTriVertices */
     int64_t num_valid_voxels; /* nValidVoxels */
@@ -198,8 +234,9 @@ typedef struct {
     double seconds_extract_device;   /* count, scan and emission on the stream,
                                         without the copy to the host */
 } its_mc_stats;
-int its_mc_extract(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
-                   double iso, float* tri_xyz, int64_t capacity, its_mc_stats* stats);
+int its_mc_extract(its_ctx ctx, its_svo svo, const double* lambda, const double origin[3], const double width[3],
+                   const int res[3], const double* iso, int n_iso, float* tri_xyz, int64_t capacity,
+                   int64_t* num_verts, its_mc_stats* stats);
 /* lattice values f at the (res+1)^3 grid points, x fastest (for checking) */
 int its_mc_lattice(its_ctx ctx, its_svo svo, const double origin[3], const double width[3], const int res[3],
                    float* values);

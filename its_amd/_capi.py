@@ -63,6 +63,21 @@ class LscgStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LscgStructure(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("nnz", C.c_int64),
+        ("parent_entries", C.c_int64),
+        ("levels", C.c_int),
+        ("block_levels", C.c_int),
+        ("bytes_assembled_per_iteration", C.c_double),
+        ("bytes_matrix_free_per_iteration", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class McStats(C.Structure):
     _fields_ = [
         ("num_verts", C.c_int64),
@@ -111,13 +126,16 @@ SIGNATURES = {
     "its_bspline_eval_f64pts": [vp, vp, _f64p, C.c_int64, _f64p, C.c_int, C.c_float, _f32p],
     "its_bspline_eval_device": [vp, vp, vp, vp, vp, C.c_int64, _f64p, C.c_int, C.c_float, vp, vp],
     "its_classify": [_f32p, C.c_int64, C.c_double, C.c_double, _i8p],
-    "its_mc_extract": [vp, vp, _f64p, _f64p, _i32p, C.c_double, vp, C.c_int64, C.POINTER(McStats)],
+    "its_mc_extract": [vp, vp, vp, _f64p, _f64p, _i32p, _f64p, C.c_int, vp, C.c_int64,
+                       np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.POINTER(McStats)],
+    "its_svo_from_vertices": [vp, _f64p, _u32p, C.c_int64, _f64p, C.c_int64, C.POINTER(vp)],
     "its_mc_lattice": [vp, vp, _f64p, _f64p, _i32p, _f32p],
     "its_tune_set": [C.c_int, C.c_int64],
     "its_tune_get": [C.c_int, C.POINTER(C.c_int64)],
     "its_timer_start": [vp],
     "its_timer_stop": [vp, C.POINTER(C.c_double)],
     "its_lscg_diag": [vp, vp, _f64p],
+    "its_lscg_structure_get": [vp, vp, C.POINTER(LscgStructure)],
     "its_shell_points": [vp, vp, _f64p, C.c_int64, _i32p, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64),
                          C.POINTER(C.c_int64)],
     "its_shell_iso": [vp, vp, _f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.POINTER(C.c_int64)],
@@ -285,6 +303,28 @@ class Octree:
         check(lib().its_svo_get_info(h, C.byref(self.info)))
         return self
 
+    @classmethod
+    def from_vertices(cls, ctx: Context, xyz, owner, node_width):
+        """Octree object holding only a vertex set (nodeVertexArray positions
+        and owners, nodeWidthArray per node as (n_nodes, 3) or (n_nodes,)):
+        evaluation and marching cubes, no nodes or solver."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        xyz = _f64(xyz).reshape(-1, 3)
+        nw = _f64(node_width)
+        if nw.ndim == 1:
+            nw = np.repeat(nw[:, None], 3, axis=1)
+        nw = np.ascontiguousarray(nw)
+        h = vp()
+        check(lib().its_svo_from_vertices(ctx.handle, xyz.reshape(-1), np.ascontiguousarray(owner, np.uint32),
+                                          len(xyz), nw.reshape(-1), len(nw), C.byref(h)))
+        self._h = h
+        self.info = SvoInfo()
+        check(lib().its_svo_get_info(h, C.byref(self.info)))
+        self.origin = np.array(self.info.grid_origin[:3])
+        self.width = np.full(3, self.info.voxel_width * self.info.R)
+        return self
+
     @property
     def handle(self):
         return self._h
@@ -420,6 +460,12 @@ class Octree:
                                   C.byref(n)))
         return float(iso[0]), float(iso[1]), n.value
 
+    def lscg_structure(self):
+        """n, assembled non-zeros, parent entries and bytes per CGLS iteration."""
+        out = LscgStructure()
+        check(lib().its_lscg_structure_get(self.ctx.handle, self._h, C.byref(out)))
+        return out.as_dict()
+
     def lscg_diag(self):
         d = np.zeros(self.num_verts)
         check(lib().its_lscg_diag(self.ctx.handle, self._h, d))
@@ -473,24 +519,36 @@ class Octree:
         check(lib().its_bspline_eval_device(self.ctx.handle, self._h, vp(dx), vp(dy), vp(dz), int(N), _f64(box_end),
                                             mode, out_val, vp(d_out), vp(stream) if stream else None))
 
-    def marching_cubes(self, origin, width, res, iso):
+    def marching_cubes(self, origin, width, res, iso, lam=None):
         """Soup (num_verts, 3) binary32 and stats.  The count call evaluates
         the lattice; the fill call reuses it, so the lattice times come from
-        the first call and the extraction times from the second."""
+        the first call and the extraction times from the second.  lam: new
+        coefficients to extract from (None: the current ones)."""
+        soups, st = self.marching_cubes_multi(origin, width, res, [iso], lam)
+        return soups[0], st
+
+    def marching_cubes_multi(self, origin, width, res, isos, lam=None):
+        """One soup per iso value from one lattice: ([soup_k], stats)."""
         res = np.asarray(res, np.int32)
+        isos = _f64(np.atleast_1d(isos))
+        counts = np.zeros(len(isos), np.int64)
         st = McStats()
-        check(lib().its_mc_extract(self.ctx.handle, self._h, _f64(origin), _f64(width), res, iso, None, 0,
-                                   C.byref(st)))
-        n = st.num_verts
+        lam_keep = _f64(lam) if lam is not None else None
+        lp = lam_keep.ctypes.data_as(vp) if lam_keep is not None else None
+        check(lib().its_mc_extract(self.ctx.handle, self._h, lp, _f64(origin), _f64(width), res, isos, len(isos),
+                                   None, 0, counts, C.byref(st)))
+        n = int(counts.sum())
         lat = (st.seconds_eval, st.seconds_eval_device)
         tri = np.zeros((max(n, 1), 3), np.float32)
         if n:
-            check(lib().its_mc_extract(self.ctx.handle, self._h, _f64(origin), _f64(width), res, iso,
-                                       tri.ctypes.data_as(vp), n, C.byref(st)))
-        return tri[:n], {"num_verts": st.num_verts, "num_valid_voxels": st.num_valid_voxels,
-                         "seconds_eval": lat[0], "seconds_extract": st.seconds_extract,
-                         "seconds_eval_device": lat[1],
-                         "seconds_extract_device": st.seconds_extract_device}
+            check(lib().its_mc_extract(self.ctx.handle, self._h, None, _f64(origin), _f64(width), res, isos,
+                                       len(isos), tri.ctypes.data_as(vp), n, counts, C.byref(st)))
+        offs = np.concatenate([[0], np.cumsum(counts)])
+        soups = [tri[offs[k]:offs[k + 1]] for k in range(len(isos))]
+        return soups, {"num_verts": st.num_verts, "num_valid_voxels": st.num_valid_voxels,
+                       "seconds_eval": lat[0], "seconds_extract": st.seconds_extract,
+                       "seconds_eval_device": lat[1],
+                       "seconds_extract_device": st.seconds_extract_device, "counts": counts}
 
     def mc_lattice(self, origin, width, res):
         res = np.asarray(res, np.int32)

@@ -276,6 +276,9 @@ void batch_delete(its_batch b) { delete b; }
 void batch_add(its_batch b, its_svo s) {
     ITS_REQUIRE(s->ctx->device == b->ctx->device, ITS_E_INVALID, "octree is on another device");
     ITS_REQUIRE(s->num_verts > 0, ITS_E_STATE, "empty octree");
+    // two workgroups (or workers) solving one octree would race on its state
+    ITS_REQUIRE(std::find(b->svos.begin(), b->svos.end(), s) == b->svos.end(), ITS_E_INVALID,
+                "octree is already in the batch");
     b->svos.push_back(s);
 }
 
@@ -313,6 +316,9 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
     b->stats.ensure(8 * (int64_t)std::max(ns, 1));
     for (int q = 0; q < ns; ++q) {
         its_svo s = b->svos[small[q]];
+        // the octree's solver state and coefficients live on its own
+        // context's stream, where fit_get launches: allocate them there
+        AllocStream as(s->ctx->stream);
         Fit* f = fit_get(s);
         lambda_changing(s);
         s->lambda64.ensure(f->n);
@@ -382,7 +388,8 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
         for (size_t q = 0; q < big.size(); ++q) {
             order[q] = q;
             its_svo s = b->svos[big[q]];
-            (void)fit_get(s);   // solver state built here, on the context stream
+            AllocStream as(s->ctx->stream);   // solver state built on the octree's own stream
+            (void)fit_get(s);
             lambda_changing(s);
             s->lambda64.ensure(s->num_verts);
         }
@@ -447,7 +454,12 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
 
 // rebuild the evaluation fields after a batch fit (per mesh)
 void batch_fields(its_batch b) {
-    for (its_svo s : b->svos) field_build(s);   // each on its octree's stream, which it synchronises
+    for (its_svo s : b->svos) {
+        // each on its octree's stream, which it synchronises; its buffers and
+        // temporaries are allocated and freed in that stream's order
+        AllocStream as(s->ctx->stream);
+        field_build(s);
+    }
 }
 
 int64_t batch_count(its_batch b) { return (int64_t)b->svos.size(); }

@@ -4,6 +4,7 @@
 #include <cfloat>
 #include <cmath>
 #include <algorithm>
+#include <atomic>
 #include <vector>
 #include <cstring>
 #include <new>
@@ -39,6 +40,9 @@ using namespace its;
     }
 
 namespace {
+// live contexts per device: the last one to go trims the device's pool
+std::atomic<int> g_live_ctx[64];
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev) {
@@ -59,6 +63,16 @@ DBuf<T> upload(const T* h, size_t n, hipStream_t st) {
 }
 
 void check_svo(its_svo s) { ITS_REQUIRE(s != nullptr && s->ctx != nullptr, ITS_E_INVALID, "null octree handle"); }
+// entry points that need nodes, voxels or the solver
+void check_full(its_svo s) {
+    check_svo(s);
+    ITS_REQUIRE(!s->vertices_only, ITS_E_STATE, "octree built from vertices has no nodes, voxels or solver");
+}
+// caller order <-> internal (level, lattice) order of per-vertex vectors
+void to_internal(its_svo s, const double* in, std::vector<double>& out) {
+    out.resize(s->num_verts);
+    for (int64_t i = 0; i < s->num_verts; ++i) out[s->vperm[i]] = in[i];
+}
 }  // namespace
 
 extern "C" {
@@ -78,7 +92,7 @@ int its_ctx_create(int device, its_ctx* out) {
     ITS_REQUIRE(out, ITS_E_INVALID, "null output pointer");
     int n = 0;
     ITS_HIP(hipGetDeviceCount(&n));
-    ITS_REQUIRE(device >= 0 && device < n, ITS_E_INVALID, "no such device");
+    ITS_REQUIRE(device >= 0 && device < n && device < 64, ITS_E_INVALID, "no such device");
     DeviceGuard g(device);
     auto* c = new its_ctx_s();
     c->device = device;
@@ -88,13 +102,10 @@ int its_ctx_create(int device, its_ctx* out) {
     ITS_HIP(hipEventCreate(&c->ev1));
     for (hipEvent_t& e : c->mc_ev) ITS_HIP(hipEventCreate(&e));
     ITS_HIP(hipGetDeviceProperties(&c->prop, device));
-    // stream-ordered allocations keep freed blocks in the device's default
-    // pool instead of returning them at every synchronisation
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-        uint64_t keep = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-    }
+    // stream-ordered allocations come from the library's own pool on this
+    // device, which keeps freed blocks; the default pool is left as it is
+    (void)device_pool();
+    g_live_ctx[device].fetch_add(1);
     *out = c;
     ITS_API_END
 }
@@ -109,7 +120,10 @@ int its_ctx_destroy(its_ctx ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->mc_ev)
         if (e) (void)hipEventDestroy(e);
+    const int dev = ctx->device;
     delete ctx;
+    // the last context of a device gives the pool's unused blocks back
+    if (g_live_ctx[dev].fetch_sub(1) == 1) device_pool_trim();
     ITS_API_END
 }
 
@@ -230,6 +244,20 @@ int its_svo_build_device(its_ctx ctx, const double* d_tris, int64_t T, const dou
     return svo_build_common(ctx, d_tris, true, T, origin, width, R, out);
 }
 
+int its_svo_from_vertices(its_ctx ctx, const double* xyz, const uint32_t* owner, int64_t n_verts,
+                          const double* node_width, int64_t n_nodes, its_svo* out) {
+    ITS_API_BEGIN
+    ITS_REQUIRE(ctx && out && xyz && owner && node_width && n_verts > 0 && n_nodes > 0, ITS_E_INVALID,
+                "bad arguments");
+    DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
+    std::unique_ptr<its_svo_s> s(new its_svo_s());
+    s->ctx = ctx;
+    svo_from_vertices(s.get(), xyz, owner, n_verts, node_width, n_nodes);
+    *out = s.release();
+    ITS_API_END
+}
+
 int its_svo_destroy(its_svo svo) {
     ITS_API_BEGIN
     if (!svo) return ITS_OK;
@@ -263,7 +291,7 @@ int its_svo_get_info(its_svo s, its_svo_info* info) {
 int its_svo_export_nodes(its_svo s, uint32_t* morton, uint8_t* is_leaf, double* origin, double* width,
                          uint32_t* parent, uint32_t* childs, uint32_t* neighbors) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     DeviceGuard g(s->ctx->device);
     AllocStream as(s->ctx->stream);
     hipStream_t st = s->ctx->stream;
@@ -317,7 +345,7 @@ int its_svo_export_vert_lattice(its_svo s, uint16_t* ijk, uint8_t* level) {
 
 int its_svo_export_fine_edges(its_svo s, double* ends, uint32_t* owner, int64_t capacity, int64_t* count) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     DeviceGuard g(s->ctx->device);
     AllocStream as(s->ctx->stream);
     hipStream_t st = s->ctx->stream;
@@ -350,7 +378,7 @@ int its_svo_export_verts(its_svo s, double* xyz, uint32_t* owner) {
 
 int its_svo_export_voxels(its_svo s, uint32_t* morton) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     DeviceGuard g(s->ctx->device);
     AllocStream as(s->ctx->stream);
     if (s->num_voxels && morton) {
@@ -364,7 +392,7 @@ int its_svo_export_voxels(its_svo s, uint32_t* morton) {
 // ---- right-hand side
 int its_sdf_at_vertices(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* b) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
     ITS_REQUIRE(V && F && nV > 0 && nF > 0, ITS_E_INVALID, "empty mesh");
     DeviceGuard g(ctx->device);
@@ -413,7 +441,7 @@ int its_signed_distance(its_ctx ctx, const double* V, int64_t nV, const int32_t*
 int its_lscg_solve(its_ctx ctx, its_svo s, const double* b, double* lambda_inout, double tol, int64_t max_iter,
                    int warm, its_lscg_stats* stats) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
     ITS_REQUIRE(lambda_inout, ITS_E_INVALID, "null lambda");
     ITS_REQUIRE(tol >= 0, ITS_E_INVALID, "negative tolerance");
@@ -444,7 +472,7 @@ int its_lscg_solve(its_ctx ctx, its_svo s, const double* b, double* lambda_inout
 
 int its_collocation_mul(its_ctx ctx, its_svo s, const double* x, double* y, int transpose) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx && x && y, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -466,9 +494,14 @@ int its_svo_set_lambda(its_ctx ctx, its_svo s, const double* lambda) {
     const int64_t n = s->num_verts;
     lambda_changing(s);
     s->lambda64.ensure(std::max<int64_t>(n, 1));
+    std::vector<double> perm;
+    if (!s->vperm.empty()) {
+        to_internal(s, lambda, perm);
+        lambda = perm.data();
+    }
     if (n) ITS_HIP(hipMemcpyAsync(s->lambda64.p, lambda, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     s->has_lambda = true;
-    field_build(s);
+    field_build(s);   // synchronises the stream before perm goes out of scope
     ITS_API_END
 }
 
@@ -479,6 +512,14 @@ int its_svo_get_lambda(its_svo s, double* lambda) {
     ITS_REQUIRE(s->has_lambda, ITS_E_STATE, "no coefficients");
     DeviceGuard g(s->ctx->device);
     AllocStream as(s->ctx->stream);
+    if (!s->vperm.empty()) {
+        std::vector<double> tmp(s->num_verts);
+        ITS_HIP(hipMemcpyAsync(tmp.data(), s->lambda64.p, s->num_verts * sizeof(double), hipMemcpyDeviceToHost,
+                               s->ctx->stream));
+        ITS_HIP(hipStreamSynchronize(s->ctx->stream));
+        for (int64_t i = 0; i < s->num_verts; ++i) lambda[i] = tmp[s->vperm[i]];
+        return ITS_OK;
+    }
     ITS_HIP(hipMemcpyAsync(lambda, s->lambda64.p, s->num_verts * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
     ITS_HIP(hipStreamSynchronize(s->ctx->stream));
     ITS_API_END
@@ -549,14 +590,38 @@ int its_classify(const float* f, int64_t N, double inner_iso, double outer_iso, 
 }
 
 // ---- marching cubes
-int its_mc_extract(its_ctx ctx, its_svo s, const double origin[3], const double width[3], const int res[3], double iso,
-                   float* tri_xyz, int64_t capacity, its_mc_stats* stats) {
+int its_mc_extract(its_ctx ctx, its_svo s, const double* lambda, const double origin[3], const double width[3],
+                   const int res[3], const double* iso, int n_iso, float* tri_xyz, int64_t capacity,
+                   int64_t* num_verts, its_mc_stats* stats) {
     ITS_API_BEGIN
     check_svo(s);
-    ITS_REQUIRE(ctx == s->ctx && origin && width && res, ITS_E_INVALID, "bad arguments");
+    ITS_REQUIRE(ctx == s->ctx && origin && width && res && iso, ITS_E_INVALID, "bad arguments");
+    ITS_REQUIRE(n_iso >= 1 && n_iso <= ITS_MC_MAX_ISO, ITS_E_INVALID, "n_iso must be in [1, ITS_MC_MAX_ISO]");
+    ITS_REQUIRE(capacity >= 0, ITS_E_INVALID, "negative capacity");
+    if (lambda) {
+        const int rc = its_svo_set_lambda(ctx, s, lambda);
+        if (rc != ITS_OK) return rc;
+    }
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
-    mc_extract(s, origin, width, res, iso, tri_xyz, capacity, stats);
+    ITS_REQUIRE(s->has_lambda, ITS_E_STATE, "no coefficients");
+    its_mc_stats total{};
+    int64_t at = 0;
+    for (int k = 0; k < n_iso; ++k) {
+        its_mc_stats S{};
+        // the lattice of the first value serves the others (same grid, same coefficients)
+        mc_extract(s, origin, width, res, iso[k], tri_xyz ? tri_xyz + 3 * at : nullptr,
+                   tri_xyz ? capacity - at : 0, &S);
+        if (num_verts) num_verts[k] = S.num_verts;
+        at += S.num_verts;
+        total.num_verts += S.num_verts;
+        total.num_valid_voxels += S.num_valid_voxels;
+        total.seconds_eval += S.seconds_eval;
+        total.seconds_extract += S.seconds_extract;
+        total.seconds_eval_device += S.seconds_eval_device;
+        total.seconds_extract_device += S.seconds_extract_device;
+    }
+    if (stats) *stats = total;
     ITS_API_END
 }
 
@@ -571,9 +636,19 @@ int its_mc_lattice(its_ctx ctx, its_svo s, const double origin[3], const double 
     ITS_API_END
 }
 
+int its_lscg_structure_get(its_ctx ctx, its_svo s, its_lscg_structure* out) {
+    ITS_API_BEGIN
+    check_full(s);
+    ITS_REQUIRE(ctx == s->ctx && out, ITS_E_INVALID, "bad arguments");
+    DeviceGuard g(ctx->device);
+    AllocStream as(ctx->stream);
+    lscg_structure(s, out);
+    ITS_API_END
+}
+
 int its_lscg_diag(its_ctx ctx, its_svo s, double* invd) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx && invd, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -633,7 +708,7 @@ ShellInput shell_upload(const double* V, int64_t nV, const int32_t* F, int64_t n
 int its_shell_points(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double* pts,
                      int64_t capacity, int64_t* n_edge, int64_t* n_face) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx, ITS_E_INVALID, "octree belongs to another context");
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -655,7 +730,7 @@ int its_shell_points(its_ctx ctx, its_svo s, const double* V, int64_t nV, const 
 int its_shell_iso(its_ctx ctx, its_svo s, const double* V, int64_t nV, const int32_t* F, int64_t nF, double iso[2],
                   int64_t* n_points) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx && iso, ITS_E_INVALID, "bad arguments");
     ITS_REQUIRE(s->field_ready, ITS_E_STATE, "no coefficient field: run a fit or set lambda first");
     DeviceGuard g(ctx->device);
@@ -699,7 +774,7 @@ int its_batch_destroy(its_batch b) {
 int its_batch_add(its_batch b, its_svo s) {
     ITS_API_BEGIN
     ITS_REQUIRE(b, ITS_E_INVALID, "null batch");
-    check_svo(s);
+    check_full(s);
     batch_add(b, s);
     ITS_API_END
 }
@@ -829,7 +904,7 @@ int its_mlp_train_step(its_mlp net, const uint16_t* d_x, const float* d_t, int64
 }
 int its_warm_features(its_ctx ctx, its_svo s, uint16_t* d_x, float* d_scale, void* stream) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx && d_x && d_scale, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -838,7 +913,7 @@ int its_warm_features(its_ctx ctx, its_svo s, uint16_t* d_x, float* d_scale, voi
 }
 int its_warm_dual(its_ctx ctx, its_svo s, const double* lambda, double* w) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx && lambda && w, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -854,7 +929,7 @@ int its_warm_dual(its_ctx ctx, its_svo s, const double* lambda, double* w) {
 }
 int its_warm_apply(its_ctx ctx, its_svo s, const double* w, double* lambda0) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     ITS_REQUIRE(ctx == s->ctx && w && lambda0, ITS_E_INVALID, "bad arguments");
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -871,7 +946,7 @@ int its_warm_apply(its_ctx ctx, its_svo s, const double* w, double* lambda0) {
 int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, double* lambda, double tol,
                        int64_t max_iter, its_lscg_stats* stats, double* seconds_predict) {
     ITS_API_BEGIN
-    check_svo(s);
+    check_full(s);
     check_mlp(net);
     ITS_REQUIRE(ctx == s->ctx && mlp_ctx(net) == ctx && lambda, ITS_E_INVALID, "bad arguments");
     ITS_REQUIRE(tol > 0, ITS_E_INVALID, "tolerance must be positive");

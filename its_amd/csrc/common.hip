@@ -1,11 +1,49 @@
 // Device-wide scans (three-phase: per-block local scan, scan of block sums,
 // add back) and the rank directory of bitsets.  wave64 shuffles do the
 // intra-wave part; one block covers 2048 items.
+#include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
 
 namespace its {
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<hipMemPool_t> g_pools[kMaxDevices];
+std::mutex g_pool_mu;
+}  // namespace
+
+hipMemPool_t device_pool() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    hipMemPool_t pool = g_pools[dev].load(std::memory_order_acquire);
+    if (pool) return pool;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    pool = g_pools[dev].load(std::memory_order_relaxed);
+    if (pool) return pool;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    if (hipMemPoolCreate(&pool, &props) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    g_pools[dev].store(pool, std::memory_order_release);
+    return pool;
+}
+
+void device_pool_trim() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return;
+    hipMemPool_t pool = g_pools[dev].load(std::memory_order_acquire);
+    if (pool) (void)hipMemPoolTrimTo(pool, 0);
+}
 
 namespace {
 constexpr int kScanThreads = 256;

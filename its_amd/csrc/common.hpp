@@ -41,6 +41,16 @@ void set_last_error(const std::string& msg);
 // synchronisation (hipFree waits for the whole device); when not set, plain
 // hipMalloc / hipFree.  Such memory is used on other streams only after a
 // synchronisation, and may be freed with hipFree once its stream is gone.
+// The library's own stream-ordered pool on the current device: created on
+// first use, kept until the process ends.  It keeps freed blocks (release
+// threshold at the maximum), so repeated temporaries are not re-mapped; the
+// device's default pool, which other code in the process may use, keeps its
+// own settings.  nullptr if the pool cannot be created (then the default
+// pool is used).
+hipMemPool_t device_pool();
+// give the unused blocks of the current device's pool back to the driver
+void device_pool_trim();
+
 inline hipStream_t& alloc_stream() {
     thread_local hipStream_t s = nullptr;
     return s;
@@ -72,7 +82,14 @@ struct DBuf {
         n = count;
         if (count) {
             hipStream_t s = alloc_stream();
-            hipError_t e = s ? hipMallocAsync((void**)&p, count * sizeof(T), s) : hipMalloc(&p, count * sizeof(T));
+            hipError_t e;
+            if (s) {
+                hipMemPool_t pool = device_pool();
+                e = pool ? hipMallocFromPoolAsync((void**)&p, count * sizeof(T), pool, s)
+                         : hipMallocAsync((void**)&p, count * sizeof(T), s);
+            } else {
+                e = hipMalloc(&p, count * sizeof(T));
+            }
             if (e != hipSuccess) {
                 p = nullptr;
                 n = 0;

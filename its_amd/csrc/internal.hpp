@@ -42,6 +42,12 @@ struct its_ctx_s {
 
 struct its_svo_s {
     its_ctx ctx = nullptr;
+    // built from a vertex set (its_svo_from_vertices): vertices, levels and
+    // the coefficient field only, no nodes, voxels or solver
+    bool vertices_only = false;
+    // vertices-only octrees: internal index of each caller vertex when the
+    // caller's order is not the level / lattice order (empty: the same order)
+    std::vector<int64_t> vperm;
     int depth = 0;
     int R = 0;
     double origin[3] = {0, 0, 0};
@@ -101,6 +107,8 @@ inline void lambda_changing(its_svo s) { ++s->lambda_version; }
 int64_t tune_get(int key);
 // svo.hip
 void svo_build(its_svo s, const double* d_tris, int64_t T);
+void svo_from_vertices(its_svo s, const double* xyz, const uint32_t* owner, int64_t n, const double* node_width,
+                       int64_t n_nodes);
 // eval.hip
 void field_build(its_svo s);   // cells + blocks from lambda64
 void dense_cascade(its_svo s, int stop, DBuf<float>& out);
@@ -118,6 +126,7 @@ void fit_release(Fit* f);
 void lscg_solve(its_svo s, const double* d_b, double* d_lambda, double tol, int64_t max_iter, bool warm,
                 its_lscg_stats* st, hipStream_t stream = nullptr);
 void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose);
+void lscg_structure(its_svo s, its_lscg_structure* out);
 // mc.hip
 void mc_extract(its_svo s, const double origin[3], const double width[3], const int res[3], double iso,
                 float* h_tri, int64_t capacity, its_mc_stats* st);

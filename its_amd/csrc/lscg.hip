@@ -792,6 +792,60 @@ void warm_apply(its_svo s, const double* d_w, double* d_lam0, hipStream_t st) {
     ITS_LAUNCH_CHECK();
 }
 
+// Non-zeros of the collocation matrix the reference assembles
+// (ThinShells.cpp:556-605): per row, the vertex itself, the finer vertices at
+// the same position (a contiguous chain down to the finest coincident one),
+// and at every coarser level the corners of the cell holding the vertex whose
+// hat is non-zero there: one corner per axis where the vertex lies on that
+// level's lattice plane, else two.
+namespace {
+__global__ void k_row_nnz(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
+                          const uint16_t* vz, const uint32_t* down, unsigned long long* total) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long c = 0;
+    if (i < n) {
+        const int d = lev[i];
+        c = 1 + (unsigned long long)(d - lev[down[i]]);
+        const uint32_t X = vx[i], Y = vy[i], Z = vz[i];
+        for (int l = d + 1; l < L; ++l) {
+            const uint32_t mask = (1u << (l - d)) - 1u;
+            c += (unsigned long long)(((X & mask) ? 2 : 1) * ((Y & mask) ? 2 : 1) * ((Z & mask) ? 2 : 1));
+        }
+    }
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+}
+}  // namespace
+
+void lscg_structure(its_svo s, its_lscg_structure* out) {
+    Fit* f = fit_get(s);
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = f->n;
+    DBuf<unsigned long long> tot(1);
+    ITS_HIP(hipMemsetAsync(tot.p, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_row_nnz, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, f->L, s->vx.p, s->vy.p,
+                       s->vz.p, (const uint32_t*)f->down.p, tot.p);
+    ITS_LAUNCH_CHECK();
+    unsigned long long nnz = 0;
+    ITS_HIP(hipMemcpyAsync(&nnz, tot.p, sizeof nnz, hipMemcpyDeviceToHost, st));
+    uint32_t entries = 0;
+    ITS_HIP(hipMemcpyAsync(&entries, f->pro_ptr.p + n, sizeof entries, hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    its_lscg_structure S{};
+    S.n = n;
+    S.nnz = (int64_t)nnz;
+    S.parent_entries = (int64_t)entries;
+    S.levels = f->L;
+    S.block_levels = f->L - f->lc;
+    S.bytes_assembled_per_iteration = 16.0 * (double)nnz + 72.0 * (double)n;
+    // per vertex: prolongation p r/w, z, list offset, g (20 B); t = D g with
+    // its index and gather (12 B); the update of x, r along the chains, h
+    // (33 B); restriction offset and h r/w (12 B); preconditioner h, 1/d, z
+    // (12 B).  Per parent entry: index and value, once down and once up (16 B).
+    S.bytes_matrix_free_per_iteration = 89.0 * (double)n + 16.0 * (double)entries;
+    *out = S;
+}
+
 void collocation_mul(its_svo s, const double* d_x, double* d_y, bool transpose) {
     Fit* f = fit_get(s);
     hipStream_t st = s->ctx->stream;

@@ -54,7 +54,8 @@ struct McGrid {
     int njb;          // ceil(ry / J)
     int64_t nwaves;   // count waves: nch * njb * rz
     int64_t nsteps;   // ry * rz * nch
-    float iso;
+    float iso;      // the cube cases compare the binary32 lattice with this
+    double iso64;   // the caller's iso value: vertexLerp's snaps and t
 };
 
 __device__ inline unsigned case8(const float (&v)[8], float iso) {
@@ -171,7 +172,7 @@ __device__ inline void emit_step(const float* __restrict__ lat, const McGrid& G,
     for (int q = 0; q < 8; ++q) sv[q] = (double)lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]];
     const uint4 tri = *reinterpret_cast<const uint4*>(&c_tri[c][0]);
     const uint64_t first = at + incl - nv;
-    const double iso = (double)G.iso;
+    const double iso = G.iso64;   // MarchingCubes.cu:72-90 works with the caller's binary64 iso
     for (unsigned t = 0; t < nv; t += 3) {
         const uint64_t pos = first + t;
         if (pos >= max_verts - 3) break;
@@ -308,6 +309,7 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     G.nwaves = (int64_t)G.nch * G.njb * res[2];
     G.nsteps = (int64_t)res[1] * res[2] * G.nch;
     G.iso = (float)iso;
+    G.iso64 = iso;
     s->mc_case.ensure(nvox);
     s->mc_step.ensure(G.nsteps);
     ITS_REQUIRE(G.nsteps < (int64_t)kStepMask, ITS_E_RANGE, "marching-cubes grid has too many rows");

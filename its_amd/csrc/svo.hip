@@ -513,4 +513,140 @@ void svo_build(its_svo s, const double* d_tris, int64_t T) {
     ITS_HIP(hipStreamSynchronize(st));
 }
 
+// An octree object that holds only a vertex set, its levels and lattices:
+// the inputs of MC::marching_cubes (MarchingCubes.h:136-139), nodeVertexArray
+// (corner position and owner node per vertex) and nodeWidthArray (per node).
+// A vertex's level follows from its owner's width, w_d = w_0 * 2^d with w_0
+// the smallest width; the grid origin is the smallest coordinate of every
+// axis (the root's first corner); a vertex's lattice key is its position in
+// units of w_d from there.  The evaluation field and marching cubes then work
+// as on a built octree: f(x) = sum_i lambda_i B_i(x) with B_i the hat of
+// width w_level(i) centred on vertex i (CUDACompute.cu:468).
+void svo_from_vertices(its_svo s, const double* xyz, const uint32_t* owner, int64_t n, const double* node_width,
+                       int64_t n_nodes) {
+    ITS_REQUIRE(n > 0, ITS_E_INVALID, "no vertices");
+    ITS_REQUIRE(n < (int64_t)UINT32_MAX, ITS_E_RANGE, "too many vertices");
+    double w0 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        ITS_REQUIRE(owner[i] < (uint64_t)n_nodes, ITS_E_INVALID, "vertex owner outside the node array");
+        const double w = node_width[3 * (int64_t)owner[i]];
+        ITS_REQUIRE(std::isfinite(w) && w > 0, ITS_E_INVALID, "node widths must be positive");
+        if (i == 0 || w < w0) w0 = w;
+    }
+    double origin[3] = {xyz[0], xyz[1], xyz[2]};
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            ITS_REQUIRE(std::isfinite(xyz[3 * i + a]), ITS_E_INVALID, "vertex coordinates must be finite");
+            origin[a] = std::min(origin[a], xyz[3 * i + a]);
+        }
+    std::vector<uint8_t> level(n);
+    int depth = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double w = node_width[3 * (int64_t)owner[i]];
+        int d = 0;
+        while (d < kMaxLevels && std::ldexp(w0, d) < w) ++d;
+        ITS_REQUIRE(d < kMaxLevels && std::ldexp(w0, d) == w, ITS_E_INVALID,
+                    "node widths must be the smallest width times a power of two");
+        level[i] = (uint8_t)d;
+        depth = std::max(depth, d + 1);
+    }
+    const int R = 1 << (depth - 1);
+    ITS_REQUIRE(R <= 1024, ITS_E_RANGE, "more than 1024 cells per axis");
+    std::vector<uint64_t> key(n);   // level in the high byte, lattice key below
+    std::vector<uint16_t> kx(n), ky(n), kz(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const int d = level[i];
+        const double w = std::ldexp(w0, d);
+        const int Rd = R >> d;
+        int64_t k[3];
+        for (int a = 0; a < 3; ++a) {
+            const double u = (xyz[3 * i + a] - origin[a]) / w;
+            k[a] = std::llround(u);
+            ITS_REQUIRE(k[a] >= 0 && k[a] <= Rd && std::fabs(u - (double)k[a]) <= 1e-6, ITS_E_INVALID,
+                        "a vertex is not a corner of its level's lattice");
+        }
+        kx[i] = (uint16_t)k[0];
+        ky[i] = (uint16_t)k[1];
+        kz[i] = (uint16_t)k[2];
+        key[i] = ((uint64_t)d << 56) | (uint64_t)((k[0] * (Rd + 1) + k[1]) * (Rd + 1) + k[2]);
+    }
+    std::vector<int64_t> ord(n);
+    for (int64_t i = 0; i < n; ++i) ord[i] = i;
+    bool sorted = true;
+    for (int64_t i = 1; i < n && sorted; ++i) sorted = key[i - 1] < key[i];
+    if (!sorted) std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    for (int64_t i = 1; i < n; ++i)
+        ITS_REQUIRE(key[ord[i - 1]] != key[ord[i]], ITS_E_INVALID, "two vertices share a level and a lattice point");
+    if (!sorted) {
+        s->vperm.assign(n, 0);
+        for (int64_t j = 0; j < n; ++j) s->vperm[ord[j]] = j;
+    }
+    s->vertices_only = true;
+    s->depth = depth;
+    s->R = R;
+    s->voxel_width = w0;
+    for (int a = 0; a < 3; ++a) {
+        s->origin[a] = origin[a];
+        s->width[a] = (double)R * w0;
+    }
+    s->num_voxels = 0;
+    s->num_nodes = 0;
+    s->num_verts = n;
+    hipStream_t st = s->ctx->stream;
+    std::vector<uint16_t> hx(n), hy(n), hz(n);
+    std::vector<uint8_t> hl(n);
+    std::vector<uint32_t> ho(n);
+    std::vector<double> hxyz(3 * n);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t i = ord[j];
+        hx[j] = kx[i];
+        hy[j] = ky[i];
+        hz[j] = kz[i];
+        hl[j] = level[i];
+        ho[j] = owner[i];
+        for (int a = 0; a < 3; ++a) hxyz[3 * j + a] = xyz[3 * i + a];
+    }
+    int64_t at = 0;
+    std::vector<std::vector<RankWord>> lat(depth);
+    for (int d = 0; d < depth; ++d) {
+        Level& L = s->lv[d];
+        L.R = R >> d;
+        L.width = std::ldexp(w0, d);
+        const int64_t D = (int64_t)L.R + 1;
+        const int64_t nwords = (D * D * D + 31) / 32;
+        std::vector<RankWord>& g = lat[d];
+        g.assign(nwords, RankWord{0, 0});
+        s->esum_verts[d] = at;
+        int64_t cnt = 0;
+        while (at < n && hl[at] == d) {
+            const uint64_t b = (uint64_t)((hx[at] * D + hy[at]) * D + hz[at]);
+            g[b >> 5].bits |= 1u << (b & 31);
+            ++at;
+            ++cnt;
+        }
+        L.n_verts = cnt;
+        uint32_t run = 0;
+        for (RankWord& w : g) {
+            w.rank = run;
+            run += (uint32_t)__builtin_popcount(w.bits);
+        }
+        L.lattice.alloc(nwords);
+        ITS_HIP(hipMemcpyAsync(L.lattice.p, g.data(), nwords * sizeof(RankWord), hipMemcpyHostToDevice, st));
+    }
+    s->esum_verts[depth] = n;
+    s->vx.alloc(n);
+    s->vy.alloc(n);
+    s->vz.alloc(n);
+    s->vlevel.alloc(n);
+    s->vowner.alloc(n);
+    s->vxyz.alloc(3 * n);
+    ITS_HIP(hipMemcpyAsync(s->vx.p, hx.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(s->vy.p, hy.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(s->vz.p, hz.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(s->vlevel.p, hl.data(), n * sizeof(uint8_t), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(s->vowner.p, ho.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(s->vxyz.p, hxyz.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipStreamSynchronize(st));   // the host staging vectors go out of scope
+}
+
 }  // namespace its

@@ -10,19 +10,27 @@
 //   ThinShells::getShellIsoVal           (reference src/ThinShells.h:126)
 //   ThinShells::multiPointQuery          (reference src/ThinShells.cpp:1183-1420)
 //   ThinShells::mcVisualization          (reference src/ThinShells.cpp:1086-1129)
-//   MC::marching_cubes                   (reference MarchingCubes.h:136-139)
+//   MC::marching_cubes                   (reference MarchingCubes.h:136-139),
+//                                         the reference's argument list, and an
+//                                         overload on the octree object
+//   ThinShells(filename, V3i), ThinShells(filename, V3i, is2UnitCube,
+//     scaleFactor)                       (reference src/ThinShells.h:52-60, 88-97)
+//   ThinShells::multiPointQuery(out_file, points)
+//                                         (reference src/ThinShells.cpp:932-974)
 // Eigen types are replaced by small value types with the same element
-// access (V3d::operator[], x(), y(), z()).  Errors from the C ABI are thrown
-// as its_host::Error.  Header only; link with its_amd/libits_hip.so.
+// access (V3d::operator[], x(), y(), z()); VXd is std::vector<double>, and
+// double3 / uint3 / make_double3 / make_uint3 are small structs of this
+// namespace.  Errors from the C ABI are thrown as its_host::Error.  Header
+// only; link with its_amd/libits_hip.so.
 //
 // Differences, by design:
 //   - creatShell computes the right-hand side, the fit, f at the model
 //     vertices and the shell iso values over the model vertices and the
 //     intersection points (its_shell_iso); the intersection points
 //     themselves stay on the device.
-//   - MC::marching_cubes takes the octree object (which holds the vertex
-//     arrays and the coefficients on the device) instead of host copies of
-//     nodeVertexArray, nodeWidthArray and lambda.
+//   - mcVisualization extracts from the octree's own device arrays through
+//     the octree overload of MC::marching_cubes; the reference-shaped
+//     overload uploads the host arrays it is given (its_svo_from_vertices).
 #pragma once
 #include <algorithm>
 #include <array>
@@ -93,6 +101,18 @@ struct SVONode {
     unsigned int neighbors[27];
 };
 using node_vertex_type = std::pair<V3d, uint32_t>;
+using VXd = std::vector<double>;
+// the CUDA vector types of MC::marching_cubes's signature
+struct double3 {
+    double x, y, z;
+};
+struct uint3 {
+    unsigned x, y, z;
+};
+inline double3 make_double3(double x, double y, double z) { return double3{x, y, z}; }
+inline double3 make_double3(const V3d& v) { return double3{v[0], v[1], v[2]}; }
+inline uint3 make_uint3(unsigned x, unsigned y, unsigned z) { return uint3{x, y, z}; }
+inline uint3 make_uint3(const V3i& v) { return uint3{(unsigned)v[0], (unsigned)v[1], (unsigned)v[2]}; }
 
 // one context per host thread, created on first use
 inline its_ctx default_ctx() {
@@ -262,6 +282,12 @@ public:
         init(V, F, is2UnitCube, scaleFactor);
     }
 
+    // ThinShells(filename, V3i) and ThinShells(filename, V3i, is2UnitCube,
+    // scaleFactor) (ThinShells.h:52-60, 88-97)
+    ThinShells(const std::string& filename, const V3i& grid) : ThinShells(filename, grid[0], grid[1], grid[2]) {}
+    ThinShells(const std::string& filename, const V3i& grid, bool is2UnitCube, double scaleFactor)
+        : ThinShells(filename, grid[0], grid[1], grid[2], is2UnitCube, scaleFactor) {}
+
     void creatShell() {
         its_ctx c = svo.ctx_();
         const int64_t n = (int64_t)svo.numNodeVerts;
@@ -381,6 +407,43 @@ public:
         std::vector<int> r = multiPointQuery({point}, t, 1);
         return r.empty() ? 0 : r[0];
     }
+    // multiPointQuery(out_file, points) (ThinShells.cpp:932-966): f of every
+    // point over all basis functions (cuAcc::cpBSplineVal, the RAW mode), as
+    // an OBJ point cloud "v x y z r g b": green (0.56471, 0.93333, 0.56471)
+    // where inner < f < outer, red (1, 0.27059, 0) elsewhere.  A name without
+    // the .obj extension gets it.
+    void multiPointQuery(const std::string& out_file, const std::vector<V3d>& points) const {
+        if (innerShellIsoVal == -kInf || outerShellIsoVal == -kInf) {
+            std::printf("Error: You must create shells first!");
+            return;
+        }
+        const int64_t N = (int64_t)points.size();
+        std::vector<double> p(3 * N);
+        for (int64_t i = 0; i < N; ++i)
+            for (int a = 0; a < 3; ++a) p[3 * i + a] = points[i][a];
+        std::vector<float> f(N);
+        const double end[3] = {bbox.boxEnd[0], bbox.boxEnd[1], bbox.boxEnd[2]};
+        if (N) check(its_bspline_eval_f64pts(svo.ctx_(), svo.handle(), p.data(), N, end, ITS_EVAL_RAW, 0.f, f.data()));
+        std::string name = out_file;
+        if (io::extension(name) != ".obj") {
+            const size_t dot = name.find_last_of('.'), slash = name.find_last_of('/');
+            if (dot != std::string::npos && (slash == std::string::npos || dot > slash)) name = name.substr(0, dot);
+            name += ".obj";
+        }
+        std::ofstream out = io::open_out(name);
+        std::printf("-- Save query result to \"%s\"-- [RED] points not on the surface, [GREEN] points lie on the "
+                    "surface\n", name.c_str());
+        for (int64_t i = 0; i < N; ++i) {
+            const double v = f[i];
+            const bool on = innerShellIsoVal < v && v < outerShellIsoVal;
+            out << "v " << points[i][0] << " " << points[i][1] << " " << points[i][2] << " "
+                << (on ? 0.56471 : 1.0) << " " << (on ? 0.93333 : 0.27059) << " " << (on ? 0.56471 : 0.0) << '\n';
+        }
+    }
+    // singlePointQuery(out_file, point) (ThinShells.cpp:901-930): the same for one point
+    void singlePointQuery(const std::string& out_file, const V3d& point) const {
+        multiPointQuery(out_file, std::vector<V3d>{point});
+    }
 
     void mcVisualization(const std::string& innerFilename, const V3i& innerResolution, const std::string& outerFilename,
                          const V3i& outerResolution, const std::string& isoFilename, const V3i& isoResolution) const;
@@ -481,25 +544,70 @@ private:
 };
 
 namespace MC {
+namespace detail {
+// soup of one octree object at one iso value (lambda: NULL = its current
+// coefficients), written as OBJ when filename is not empty
+inline int64_t extract(its_ctx ctx, its_svo svo, const double* lambda, const double o[3], const double w[3],
+                       const int res[3], double isoVal, const std::string& filename) {
+    its_mc_stats st;
+    int64_t nv = 0;
+    check(its_mc_extract(ctx, svo, lambda, o, w, res, &isoVal, 1, nullptr, 0, &nv, &st));
+    std::vector<float> tri(3 * (size_t)nv);
+    if (nv) check(its_mc_extract(ctx, svo, nullptr, o, w, res, &isoVal, 1, tri.data(), nv, &nv, &st));
+    if (!filename.empty()) {
+        std::vector<io::P3> soup(nv);
+        for (int64_t i = 0; i < nv; ++i) soup[i] = {tri[3 * i], tri[3 * i + 1], tri[3 * i + 2]};
+        io::save_soup_obj(filename, soup);
+    }
+    return nv;
+}
+}  // namespace detail
+
 // Triangle soup of the iso-surface over gridOrigin + [0, gridWidth) at
 // `resolution` voxels, written as OBJ (one vertex per corner, faces 3 by 3,
-// the soup order of MarchingCubes.cu:635-661).  Returns the vertex count.
+// the soup order of MarchingCubes.cu:635-661), from the octree's own arrays
+// and current coefficients.  Returns the vertex count.
 inline int64_t marching_cubes(const SparseVoxelOctree& svo, const V3d& gridOrigin, const V3d& gridWidth,
                               const V3i& resolution, double isoVal, const std::string& filename) {
     const double o[3] = {gridOrigin[0], gridOrigin[1], gridOrigin[2]};
     const double w[3] = {gridWidth[0], gridWidth[1], gridWidth[2]};
     const int res[3] = {resolution[0], resolution[1], resolution[2]};
-    its_mc_stats st;
-    check(its_mc_extract(svo.ctx_(), svo.handle(), o, w, res, isoVal, nullptr, 0, &st));
-    std::vector<float> tri(3 * (size_t)st.num_verts);
-    if (st.num_verts)
-        check(its_mc_extract(svo.ctx_(), svo.handle(), o, w, res, isoVal, tri.data(), st.num_verts, &st));
-    if (!filename.empty()) {
-        std::vector<io::P3> soup(st.num_verts);
-        for (int64_t i = 0; i < st.num_verts; ++i) soup[i] = {tri[3 * i], tri[3 * i + 1], tri[3 * i + 2]};
-        io::save_soup_obj(filename, soup);
+    return detail::extract(svo.ctx_(), svo.handle(), nullptr, o, w, res, isoVal, filename);
+}
+
+// The reference's MC::marching_cubes (MarchingCubes.h:136-139): the field
+// f(x) = sum_i lambda_i B_i(x) of the vertices nodeVertexArray (position,
+// owner node) with the widths nodeWidthArray of their owners, on the grid
+// gridOrigin + [0, gridWidth) of `resolution` voxels, iso value isoVal; the
+// soup is written to `filename` as OBJ.  The arrays are uploaded on each call
+// (its_svo_from_vertices) and released after it.
+inline void marching_cubes(const std::vector<node_vertex_type>& nodeVertexArray, const size_t& numNodes,
+                           const std::vector<V3d>& nodeWidthArray, const size_t& numNodeVerts, const VXd& lambda,
+                           const double3& gridOrigin, const double3& gridWidth, const uint3& resolution,
+                           const double& isoVal, const std::string& filename) {
+    if (nodeVertexArray.size() < numNodeVerts || nodeWidthArray.size() < numNodes || lambda.size() < numNodeVerts)
+        throw Error(ITS_E_INVALID, "MC::marching_cubes: arrays shorter than numNodeVerts / numNodes");
+    std::vector<double> xyz(3 * numNodeVerts), nw(3 * numNodes);
+    std::vector<uint32_t> owner(numNodeVerts);
+    for (size_t i = 0; i < numNodeVerts; ++i) {
+        for (int a = 0; a < 3; ++a) xyz[3 * i + a] = nodeVertexArray[i].first[a];
+        owner[i] = nodeVertexArray[i].second;
     }
-    return st.num_verts;
+    for (size_t j = 0; j < numNodes; ++j)
+        for (int a = 0; a < 3; ++a) nw[3 * j + a] = nodeWidthArray[j][a];
+    its_ctx ctx = default_ctx();
+    its_svo s = nullptr;
+    check(its_svo_from_vertices(ctx, xyz.data(), owner.data(), (int64_t)numNodeVerts, nw.data(), (int64_t)numNodes,
+                                &s));
+    struct Release {
+        its_svo s;
+        ~Release() { its_svo_destroy(s); }
+    } release{s};
+    const double o[3] = {gridOrigin.x, gridOrigin.y, gridOrigin.z};
+    const double w[3] = {gridWidth.x, gridWidth.y, gridWidth.z};
+    const int res[3] = {(int)resolution.x, (int)resolution.y, (int)resolution.z};
+    const int64_t nv = detail::extract(ctx, s, lambda.data(), o, w, res, isoVal, filename);
+    std::printf("-- [MC] %lld soup vertices at iso %.9g\n", (long long)nv, isoVal);
 }
 }  // namespace MC
 

@@ -1136,12 +1136,17 @@ static void voxel_corners(const double o[3], const double vs[3], int64_t ix, int
 
 /* MC::marching_cubes (MarchingCubes.cu:672-714): corner values by the dense
  * evaluation of every voxel's 8 corners, cube index bit i <=> f_i < iso,
- * soup in voxel-linear order (x fastest), triangles in table order. */
-ora_mc* ora_marching_cubes(const ora_svo* s, const double* lambda, const double origin[3], const double width[3],
-                           const int res[3], double iso) {
+ * soup in voxel-linear order (x fastest), triangles in table order.
+ * The work is restricted to the voxels of the z-slab [z0, z1) of the grid;
+ * positions are those of the whole grid, so a slab's soup is the contiguous
+ * piece of the whole grid's soup that those voxels emit. */
+static ora_mc* mc_slab(const ora_svo* s, const double* lambda, const double origin[3], const double width[3],
+                       const int res[3], double iso, int z0, int z1) {
     mc_tables();
     ora_mc* m = (ora_mc*)calloc(1, sizeof(ora_mc));
-    int64_t nvox = (int64_t)res[0] * res[1] * res[2];
+    const int64_t plane = (int64_t)res[0] * res[1];
+    const int64_t v0 = plane * z0;
+    int64_t nvox = plane * (z1 - z0);
     m->nvox = nvox;
     m->cube = (uint8_t*)calloc(nvox ? nvox : 1, 1);
     double vs[3] = {width[0] / res[0], width[1] / res[1], width[2] / res[2]};
@@ -1152,7 +1157,8 @@ ora_mc* ora_marching_cubes(const ora_svo* s, const double* lambda, const double 
     for (int64_t b = 0; b < nvox; b += chunk) {
         int64_t e = b + chunk < nvox ? b + chunk : nvox;
         for (int64_t v = b; v < e; ++v) {
-            int64_t ix = v % res[0], iy = v % ((int64_t)res[0] * res[1]) / res[0], iz = v / ((int64_t)res[0] * res[1]);
+            int64_t g = v + v0;
+            int64_t ix = g % res[0], iy = g % plane / res[0], iz = g / plane;
             d3 c[8];
             voxel_corners(origin, vs, ix, iy, iz, c);
             for (int k = 0; k < 8; ++k) {
@@ -1183,7 +1189,8 @@ ora_mc* ora_marching_cubes(const ora_svo* s, const double* lambda, const double 
         int c = m->cube[v];
         int nv = g_nverts[c];
         if (!nv) continue;
-        int64_t ix = v % res[0], iy = v % ((int64_t)res[0] * res[1]) / res[0], iz = v / ((int64_t)res[0] * res[1]);
+        int64_t g = v + v0;
+        int64_t ix = g % res[0], iy = g % plane / res[0], iz = g / plane;
         /* voxelToMeshKernel recomputes voxelPos = shift * voxelSize + origin */
         double vp[3] = {(double)ix * vs[0] + origin[0], (double)iy * vs[1] + origin[1], (double)iz * vs[2] + origin[2]};
         d3 cr[8];
@@ -1210,6 +1217,17 @@ ora_mc* ora_marching_cubes(const ora_svo* s, const double* lambda, const double 
     free(vscan);
     free(sdf);
     return m;
+}
+ora_mc* ora_marching_cubes(const ora_svo* s, const double* lambda, const double origin[3], const double width[3],
+                           const int res[3], double iso) {
+    return mc_slab(s, lambda, origin, width, res, iso, 0, res[2]);
+}
+ora_mc* ora_marching_cubes_slab(const ora_svo* s, const double* lambda, const double origin[3],
+                                const double width[3], const int res[3], double iso, int z0, int z1) {
+    if (z0 < 0) z0 = 0;
+    if (z1 > res[2]) z1 = res[2];
+    if (z1 < z0) z1 = z0;
+    return mc_slab(s, lambda, origin, width, res, iso, z0, z1);
 }
 void ora_mc_free(ora_mc* m) { if (!m) return; free(m->soup); free(m->cube); free(m); }
 int64_t ora_mc_num_verts(const ora_mc* m) { return m->nverts; }

@@ -106,6 +106,10 @@ void ora_eval_dense_fast(const ora_svo* s, const double* lambda, const double* p
 typedef struct ora_mc ora_mc;
 ora_mc* ora_marching_cubes(const ora_svo* s, const double* lambda, const double origin[3], const double width[3],
                            const int res[3], double iso);
+/* the voxels of the z-slab [z0, z1) of the same grid only: their cube
+ * indices and the piece of the whole grid's soup they emit */
+ora_mc* ora_marching_cubes_slab(const ora_svo* s, const double* lambda, const double origin[3],
+                                const double width[3], const int res[3], double iso, int z0, int z1);
 void ora_mc_free(ora_mc* m);
 int64_t ora_mc_num_verts(const ora_mc* m);           /* soup vertices (3 per triangle) */
 int64_t ora_mc_num_valid_voxels(const ora_mc* m);

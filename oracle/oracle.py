@@ -65,6 +65,7 @@ def lib():
             "ora_eval_dense": (None, [vp, _f64p, _f64p, C.c_int64, _f64p]),
             "ora_eval_dense_fast": (None, [vp, _f64p, _f64p, C.c_int64, _f64p]),
             "ora_marching_cubes": (vp, [vp, _f64p, _f64p, _f64p, _i32p, C.c_double]),
+            "ora_marching_cubes_slab": (vp, [vp, _f64p, _f64p, _f64p, _i32p, C.c_double, C.c_int, C.c_int]),
             "ora_mc_free": (None, [vp]),
             "ora_mc_num_verts": (C.c_int64, [vp]),
             "ora_mc_num_valid_voxels": (C.c_int64, [vp]),
@@ -283,16 +284,26 @@ def lscg_operator(mul, tmul, invdiag, b, tol=1e-6, max_iter=None, x0=None):
                "residual_abs": float(np.linalg.norm(r))}
 
 
-def marching_cubes(svo: SVO, lam, origin, width, res, iso):
+def marching_cubes(svo: SVO, lam, origin, width, res, iso, zslab=None):
+    """MC::marching_cubes restated: (soup, cube indices, valid voxels).  With
+    zslab=(z0, z1) only the voxels of that z-range of the same grid: their
+    cube indices and the contiguous piece of the whole soup they emit."""
     L = lib()
-    h = L.ora_marching_cubes(svo._h, _f64(lam), _f64(origin), _f64(width), np.asarray(res, np.int32), iso)
+    res = np.asarray(res, np.int32)
+    if zslab is None:
+        h = L.ora_marching_cubes(svo._h, _f64(lam), _f64(origin), _f64(width), res, iso)
+        nvox = int(np.prod(res))
+    else:
+        z0, z1 = int(zslab[0]), int(zslab[1])
+        h = L.ora_marching_cubes_slab(svo._h, _f64(lam), _f64(origin), _f64(width), res, iso, z0, z1)
+        nvox = int(res[0]) * int(res[1]) * (z1 - z0)
     try:
         nv = L.ora_mc_num_verts(h)
         soup = np.zeros((max(nv, 1), 3))
         L.ora_mc_soup(h, soup.reshape(-1))
-        cube = np.zeros(int(np.prod(res)), np.uint8)
+        cube = np.zeros(max(nvox, 1), np.uint8)
         L.ora_mc_cube_index(h, cube)
-        return soup[:nv], cube, L.ora_mc_num_valid_voxels(h)
+        return soup[:nv], cube[:nvox], L.ora_mc_num_valid_voxels(h)
     finally:
         L.ora_mc_free(h)
 

@@ -93,3 +93,40 @@ def fitted(name, R):
     return Fitted(case(name, R))
 
 REFERENCE_SRC = "/root/reference/src"
+
+
+# setUniformBoundingBox (BaseModel.cpp:24-51) works partly in binary32: the
+# longest side is found with the C fmaxf on the sides rounded to float, a side
+# is skipped only when that float equals it exactly, the padding delta is a
+# float, halved in float; the final 1/11 margin is binary64.  The expected
+# boxes below are derived by hand from those rules.
+F32_09 = 15099494 / 2 ** 24          # binary32 nearest to 0.9 (0.9 * 2^24 = 15099494.4)
+
+
+def bbox_known_answers():
+    """[(V, origin, end, width)] of setUniformBoundingBox for three point sets."""
+    cases = []
+    # (a) sides 1, 1/2, 1/4, all exact in binary32: x skipped, y padded by 1/4, z by 3/8
+    V = np.array([[0.0, 0.0, 0.0], [1.0, 0.5, 0.25]])
+    o = np.array([0.0, -0.25, -0.375])
+    e = np.array([1.0, 0.75, 0.625])
+    cases.append((V, o, e))
+    # (b) side 0.1 is not a binary32 number: delta = float(1 - 0.1) = float(0.9),
+    #     halved in binary32; a binary64 delta would give 0.45
+    V = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 1.0]])
+    o = np.array([0.0, 0.0 - F32_09 / 2, 0.0])
+    e = np.array([1.0, 0.1 + F32_09 / 2, 1.0])
+    cases.append((V, o, e))
+    # (c) the longest side 1 + 2^-30 rounds to 1.0f, which differs from it, so
+    #     that side gets delta = float(1 - (1 + 2^-30)) = -2^-30: it shrinks by
+    #     2^-31 at each end; the side 1 is exact and skipped
+    V = np.array([[0.0, 0.0, 0.0], [1.0 + 2.0 ** -30, 1.0, 0.5]])
+    o = np.array([2.0 ** -31, 0.0, -0.25])
+    e = np.array([1.0 + 2.0 ** -30 - 2.0 ** -31, 1.0, 0.75])
+    cases.append((V, o, e))
+    out = []
+    for V, o, e in cases:
+        eps = (e - o) / 11
+        out.append((V, o - eps, e + eps, (e + eps) - (o - eps)))
+    return out
+

@@ -72,6 +72,17 @@ def test_unit_cube_and_bbox_bit_equal(seed):
         assert np.array_equal(np.asarray(got), np.asarray(want))
 
 
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_bbox_known_answers(k):
+    """The library's setUniformBoundingBox on the hand-derived boxes
+    (BaseModel.cpp:24-51: binary32 longest side and padding, binary64 1/11)."""
+    from helpers import bbox_known_answers
+    V, o, e, w = bbox_known_answers()[k]
+    go, ge, gw = _capi.uniform_bbox(V)
+    for got, want in ((go, o), (ge, e), (gw, w)):
+        np.testing.assert_array_equal(np.asarray(got).view(np.uint64), want.view(np.uint64))
+
+
 def test_unit_cube_known_answer():
     # a box [2,4]x[1,2]x[0,1]: scale 2, centre (3,1.5,0.5) maps to (0.5,0.5,0.5)
     # and the longest side to [0,1]

@@ -58,6 +58,7 @@ def test_batch_fit_and_eval_match_single(ctx, cap):
         Q[offs[m]:offs[m + 1]] = rng.uniform(c.origin - 0.05, c.end + 0.05, size=(counts[m], 3))
     dq = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
     df = torch.full((max(offs[-1], 1),), float("nan"), device="cuda")
+    torch.cuda.synchronize()   # the fill runs on torch's stream, the evaluation on the context's
     batch.eval_device(dq[0].data_ptr(), dq[1].data_ptr(), dq[2].data_ptr(), offs, df.data_ptr(), 0.5)
     fb = df.cpu().numpy()
     _capi.tune_set(_capi.TUNE_EVAL_QUEUE, 0)
@@ -86,6 +87,8 @@ def test_batch_warm_and_errors(ctx):
         batch.fit()                              # empty batch
     for _, g, _ in items:
         batch.add(g)
+    with pytest.raises(_capi.ItsError):
+        batch.add(items[0][1])                   # the same octree twice would race in the fit
     st0, _ = batch.fit(tol=1e-6)
     st1, _ = batch.fit(tol=1e-6, warm=True)      # from the converged coefficients
     for a, b in zip(st0, st1):
@@ -132,13 +135,28 @@ def test_batch_from_thread_contexts(ctx):
                         for k, n in zip(MESHES, counts)]).astype(np.float32)
     dq = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
 
+    _, busy, busy_b = _build(ctx)[1]
+    busy.lscg(busy_b, tol=1e-6)
+    nb = 4_000_000
+    qb = torch.rand((3, nb), device="cuda")
+    fbusy = torch.empty(nb, device="cuda")
+    torch.cuda.synchronize()
+
     def fit_eval(items):
         batch = _capi.Batch(ctx)
         for _, g, _ in items:
             batch.add(g)
+        # queued, unsynchronised work on the batch context's stream when the fit starts
+        for _ in range(4):
+            busy.eval_device(qb[0].data_ptr(), qb[1].data_ptr(), qb[2].data_ptr(), nb, busy.origin + busy.width,
+                             fbusy.data_ptr())
         st, _ = batch.fit(tol=1e-6)
+        for _ in range(4):
+            busy.eval_device(qb[0].data_ptr(), qb[1].data_ptr(), qb[2].data_ptr(), nb, busy.origin + busy.width,
+                             fbusy.data_ptr())
         batch.fields()
         df = torch.zeros(offs[-1], device="cuda")
+        torch.cuda.synchronize()   # the fill runs on torch's stream, the evaluation on the context's
         batch.eval_device(dq[0].data_ptr(), dq[1].data_ptr(), dq[2].data_ptr(), offs, df.data_ptr(), 0.5)
         lams = [g.get_lambda() for _, g, _ in items]
         batch.close()

@@ -68,3 +68,15 @@ def test_lscg_zero_rhs(ctx):
     g = _gpu(ctx, F)
     lam, st = g.lscg(np.zeros(g.num_verts), tol=1e-6)
     assert np.all(lam == 0)
+
+
+@pytest.mark.parametrize("name,R", [("icosphere", 64), ("bunny", 32), ("torus", 32), ("single", 8), ("corner", 16)])
+def test_assembled_nnz_matches_oracle(ctx, name, R):
+    """The library's count of the reference matrix's non-zeros (for the
+    assembled-matrix byte count of an iteration) equals the oracle's
+    assembled CSR (ThinShells.cpp:556-605)."""
+    F = fitted(name, R)
+    g = _gpu(ctx, F)
+    st = g.lscg_structure()
+    assert st["n"] == F.A.n and st["nnz"] == F.A.nnz
+    assert st["bytes_assembled_per_iteration"] == 16.0 * F.A.nnz + 72.0 * F.A.n

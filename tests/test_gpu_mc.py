@@ -129,3 +129,74 @@ def test_mc_lattice_follows_coefficients(ctx, res):
     lam3, _ = g.lscg(F.b, tol=1e-6)
     lat3 = g.mc_lattice(c.origin, c.width, R3)
     assert np.abs(lat3 - F.svo.eval_dense_fast(lam3, P)).max() <= 1e-5
+
+
+def test_mc_several_iso_values_and_coefficients(ctx):
+    """its_mc_extract with an iso list and new coefficients (the three
+    extractions of mcVisualization, ThinShells.cpp:1086-1129): the soups
+    equal one extraction per value, in order."""
+    F = fitted("icosphere", 64)
+    c = F.case
+    from its_amd import _capi
+    g = _capi.Octree(ctx, c.tris, c.origin, c.width, c.R)
+    isos = [-0.01, 0.0, 0.015]
+    soups, st = g.marching_cubes_multi(c.origin, c.width, (48, 48, 48), isos, lam=F.lam)
+    assert st["num_verts"] == sum(len(s) for s in soups)
+    for iso, soup in zip(isos, soups):
+        one, _ = g.marching_cubes(c.origin, c.width, (48, 48, 48), iso)
+        assert np.array_equal(one, soup)
+    # the coefficients passed in are the octree's from then on
+    np.testing.assert_array_equal(g.get_lambda(), F.lam)
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_octree_from_vertices(ctx, shuffle):
+    """An octree object from the vertex arrays MC::marching_cubes takes
+    (nodeVertexArray, nodeWidthArray; MarchingCubes.h:136-139): levels from
+    the owner widths, lattice keys from the positions.  Evaluation and the
+    soup are bit-identical to the octree the arrays came from, in the
+    reference's vertex order and in a shuffled one."""
+    from its_amd import _capi
+    F = fitted("icosphere", 64)
+    c = F.case
+    g = _gpu(ctx, F)
+    xyz, owner = g.verts()
+    widths = g.nodes()["width"]
+    lam = F.lam
+    if shuffle:
+        perm = np.random.default_rng(5).permutation(len(xyz))
+        xyz, owner, lam = xyz[perm], owner[perm], lam[perm]
+    v = _capi.Octree.from_vertices(ctx, xyz, owner, widths)
+    assert v.num_verts == g.num_verts and v.depth == g.depth
+    np.testing.assert_array_equal(v.origin.view(np.uint64), c.origin.view(np.uint64))
+    v.set_lambda(lam)
+    np.testing.assert_array_equal(v.get_lambda(), lam)
+    q = np.random.default_rng(2).uniform(c.origin - 0.05, c.end + 0.05, size=(20000, 3)).astype(np.float32)
+    for mode in (_capi.EVAL_RAW, _capi.EVAL_DENSE, _capi.EVAL_SUPPORT):
+        np.testing.assert_array_equal(v.eval(q, c.end, mode, 0.25), g.eval(q, c.end, mode, 0.25))
+    a, _ = v.marching_cubes(c.origin, c.width, (64, 64, 64), 0.0)
+    b, _ = g.marching_cubes(c.origin, c.width, (64, 64, 64), 0.0)
+    assert len(a) > 0
+    np.testing.assert_array_equal(a, b)
+    with pytest.raises(_capi.ItsError):
+        v.lscg(np.zeros(v.num_verts))          # no solver on a vertex-only octree
+    v.close()
+
+
+def test_octree_from_vertices_errors(ctx):
+    from its_amd import _capi
+    F = fitted("icosphere", 16)
+    g = _gpu(ctx, F)
+    xyz, owner = g.verts()
+    widths = g.nodes()["width"].copy()
+    bad = widths.copy()
+    bad[owner[0]] *= 1.5                        # not the finest width times a power of two
+    with pytest.raises(_capi.ItsError):
+        _capi.Octree.from_vertices(ctx, xyz, owner, bad)
+    dup = np.concatenate([xyz, xyz[:1]])        # a vertex twice on one level
+    with pytest.raises(_capi.ItsError):
+        _capi.Octree.from_vertices(ctx, dup, np.concatenate([owner, owner[:1]]), widths)
+    off = xyz.copy()
+    off[-1] += 0.3 * widths[owner[-1]]          # off its level's lattice
+    with pytest.raises(_capi.ItsError):
+        _capi.Octree.from_vertices(ctx, off, owner, widths)

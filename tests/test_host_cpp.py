@@ -15,6 +15,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DEMO = os.path.join(ROOT, "its_amd", "host", "shell_demo")
+REF_CALLER = os.path.join(ROOT, "its_amd", "host", "ref_caller")
 GOLDEN = os.path.join(ROOT, "tests", "golden", "bunny.npz")
 
 
@@ -145,4 +146,59 @@ def test_demo_dumps_and_validation(ctx, tmp_path):
     assert got["check_rate"] == pytest.approx(100.0 * correct / nq, abs=1e-6)
     assert len(_read(dump / "query_err.xyz").splitlines()) == nq - correct
     assert correct > nq // 2
+    svo.close()
+
+
+def _read_obj_vertices(path):
+    with open(path) as fh:
+        return np.array([[float(x) for x in l.split()[1:]] for l in fh if l.startswith("v ")])
+
+
+@pytest.mark.gpu
+def test_reference_shaped_caller(ctx, tmp_path):
+    """its_amd/host/ref_caller.cpp calls the host classes with the
+    reference's argument lists (ThinShells(file, V3i, is2UnitCube, scale),
+    mcVisualization, MC::marching_cubes(nodeVertexArray, numNodes,
+    nodeWidthArray, numNodeVerts, lambda, double3, double3, uint3, iso, file),
+    multiPointQuery(out_file, points)).  Its files are checked against the
+    same steps through the Python binding."""
+    from its_amd import _capi, meshes
+    d = np.load(GOLDEN)
+    V, F = d["V"], d["F"].astype(np.int32)
+    off = tmp_path / "bunny.off"
+    _write_off(off, V, F)
+    R, res = 32, 40
+    out = tmp_path / "out"
+    print(f"[ref caller] bunny R={R} res={res}", flush=True)
+    r = subprocess.run([REF_CALLER, str(off), str(R), str(res), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    Vn = _capi.model_to_unit_cube(V)
+    origin, end, width = _capi.uniform_bbox(Vn)
+    svo = _capi.Octree(ctx, meshes.triangles(Vn, F), origin, width, R)
+    b = svo.sdf_at_vertices(Vn, F)
+    lam, _ = svo.lscg(b, tol=1e-6)
+    inner, outer, _ = svo.shell_iso(Vn, F)
+    assert got["num_verts"] == svo.num_verts and got["tree_depth"] == svo.depth
+    assert got["inner_iso"] == inner and got["outer_iso"] == outer
+    soups, _ = svo.marching_cubes_multi(origin, width, (res, res, res), [inner, outer, 0.0])
+    for name, soup in zip(["mc_innerShell.obj", "mc_outerShell.obj", "mc_isosurface.obj"], soups):
+        P = _read_obj_vertices(out / name)
+        assert P.shape == soup.shape and len(P) > 0, name
+        assert np.abs(P - soup).max() <= 1e-5, name          # text at 6 significant digits
+    # the reference-signature overload: the same soup from the host arrays
+    P = _read_obj_vertices(out / "mc_reference_signature.obj")
+    assert P.shape == soups[2].shape and np.abs(P - soups[2]).max() <= 1e-5
+    # query point cloud, coloured by inner < f < outer over all basis functions
+    Q = _read_obj_vertices(out / "query_result.obj")
+    assert Q.shape == (200, 6)
+    i = np.arange(1, 201, dtype=np.float64)[:, None]
+    a = np.arange(3, dtype=np.float64)[None, :]
+    pts = origin + width * np.fmod(0.7548776662466927 * i * (a + 1) + 0.2 * a, 1.0)   # the caller's points
+    assert np.abs(Q[:, :3] - pts).max() <= 1e-5
+    f = svo.eval(pts, end, _capi.EVAL_RAW, 0.0)
+    on = (inner < f) & (f < outer)
+    np.testing.assert_allclose(Q[on, 3:], np.tile([0.56471, 0.93333, 0.56471], (on.sum(), 1)))
+    np.testing.assert_allclose(Q[~on, 3:], np.tile([1, 0.27059, 0], ((~on).sum(), 1)))
+    assert _read_obj_vertices(out / "single_query.obj").shape == (1, 6)
     svo.close()
