@@ -92,18 +92,26 @@ def test_error_codes(ctx):
     L = _capi.lib()
     res = np.array([16, 16, 16], np.int32)
     st = _capi.McStats()
-    assert L.its_mc_extract(ctx.handle, g.handle, c.origin, c.width, res, 0.0, None, 0, C.byref(st)) == 0
+    iso = np.zeros(1)
+    nv = np.zeros(1, np.int64)
+    assert L.its_mc_extract(ctx.handle, g.handle, None, c.origin, c.width, res, iso, 1, None, 0, nv,
+                            C.byref(st)) == 0
     n = st.num_verts
-    assert n > 3
+    assert n > 3 and nv[0] == n
     buf = np.zeros((n, 3), np.float32)
     # one vertex short: ITS_E_RANGE, nothing written
-    rc = L.its_mc_extract(ctx.handle, g.handle, c.origin, c.width, res, 0.0, buf.ctypes.data_as(C.c_void_p), n - 1,
-                          C.byref(st))
+    rc = L.its_mc_extract(ctx.handle, g.handle, None, c.origin, c.width, res, iso, 1, buf.ctypes.data_as(C.c_void_p),
+                          n - 1, nv, C.byref(st))
     assert rc == E_RANGE and not buf.any()
     # a resolution of zero: ITS_E_INVALID
-    rc = L.its_mc_extract(ctx.handle, g.handle, c.origin, c.width, np.array([16, 0, 16], np.int32), 0.0, None, 0,
-                          C.byref(st))
+    rc = L.its_mc_extract(ctx.handle, g.handle, None, c.origin, c.width, np.array([16, 0, 16], np.int32), iso, 1,
+                          None, 0, nv, C.byref(st))
     assert rc == E_INVALID
+    # no iso value, or more than ITS_MC_MAX_ISO: ITS_E_INVALID
+    for k in (0, 17):
+        rc = L.its_mc_extract(ctx.handle, g.handle, None, c.origin, c.width, res, np.zeros(max(k, 1)), k, None, 0,
+                              np.zeros(max(k, 1), np.int64), C.byref(st))
+        assert rc == E_INVALID, k
     # the same call afterwards still works
     tri, st2 = g.marching_cubes(c.origin, c.width, (16, 16, 16), 0.0)
     assert st2["num_verts"] == n
