@@ -795,9 +795,9 @@ int its_batch_fields(its_batch b) {
     batch_fields(b);
     ITS_API_END
 }
-int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const float* d_z, const int64_t* offsets,
-                          float out_val, float* d_f, double* seconds) {
-    ITS_API_BEGIN
+// argument checks shared by both batched evaluation entry points
+static void check_batch_eval(its_batch b, const float* d_x, const float* d_y, const float* d_z,
+                             const int64_t* offsets, float* d_f) {
     ITS_REQUIRE(b && offsets, ITS_E_INVALID, "bad arguments");
     const int64_t count = batch_count(b);
     ITS_REQUIRE(count > 0, ITS_E_STATE, "empty batch");
@@ -807,6 +807,13 @@ int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const
         ITS_REQUIRE(batch_svo(b, m)->field_ready, ITS_E_STATE, "a mesh has no evaluation field (its_batch_fields)");
     }
     ITS_REQUIRE(offsets[count] == 0 || (d_x && d_y && d_z && d_f), ITS_E_INVALID, "null query arrays");
+}
+
+int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const float* d_z, const int64_t* offsets,
+                          float out_val, float* d_f, double* seconds) {
+    ITS_API_BEGIN
+    check_batch_eval(b, d_x, d_y, d_z, offsets, d_f);
+    const int64_t count = batch_count(b);
     its_ctx ctx = batch_ctx(b);
     DeviceGuard g(ctx->device);
     AllocStream as(ctx->stream);
@@ -821,6 +828,7 @@ int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const
     if (seconds) *seconds = ms * 1e-3;
     ITS_API_END
 }
+
 
 // ---- learned warm start
 static void check_mlp(its_mlp net) { ITS_REQUIRE(net, ITS_E_INVALID, "null network"); }

@@ -422,21 +422,32 @@ __global__ __launch_bounds__(kT) void k_eval_batch(const EvalParams* P, const Ba
 //
 // Random queries diverge: most stop at the collapsed level, a few walk every
 // fine level, and a wavefront runs the longest walk of its 64 lanes.  Here
-// every wavefront evaluates the collapsed level for 64 queries at a time and
-// pushes only the queries with a marked child into an LDS stack of its own.
-// Whenever the stack holds 64 entries the wavefront walks the fine levels for
-// exactly those 64, so the fine-level instructions run on full wavefronts.
+// every wavefront evaluates the collapsed level for 64 queries at a time (a
+// chunk) and pushes only the queries with a marked child into a FIFO of its
+// own in LDS.  Whenever the FIFO holds 64 entries the wavefront walks the fine
+// levels for the oldest 64, so the fine-level instructions run on full
+// wavefronts.
 //
-// Entries are the query index within the launch (at most 2^31 queries per
-// launch), the grid coordinates u, the partial sum and the collapsed cell's
-// link word.  Points on the upper faces (u = R) take the clamped walk
-// with everything outside the grid.
+// Each output line is written once.  A chunk whose queries all stopped at the
+// collapsed level is stored at once; otherwise its 64 results wait in one of
+// kSlots LDS slots until its last queued query has been walked, and the whole
+// chunk is stored then.  Slots are reused round-robin; the FIFO hands out the
+// oldest chunk's entries first, so a slot that is still waiting when its turn
+// comes is freed by walking the FIFO's head (with fewer than 64 entries if
+// need be).
+//
+// Entries are the grid coordinates u, the partial sum, the collapsed cell's
+// link word and the result's place (slot, lane).  Points on the upper faces
+// (u = R) take the clamped walk with everything outside the grid.
 constexpr int kWaves = kT / 64;
 constexpr int kQueue = 128;   // at most 63 pending + 64 new entries
+constexpr int kSlots = 4;     // chunks waiting for fine walks, per wavefront
 
 struct WaveQueue {
     float ux[kWaves][kQueue], uy[kWaves][kQueue], uz[kWaves][kQueue], f[kWaves][kQueue];
-    uint32_t i[kWaves][kQueue], base[kWaves][kQueue], mask[kWaves][kQueue];
+    uint32_t tag[kWaves][kQueue], base[kWaves][kQueue], mask[kWaves][kQueue];
+    float res[kWaves][kSlots][64];
+    int64_t start[kWaves][kSlots];   // first query of the chunk waiting in each slot
 };
 
 __device__ inline void lds_fence() {
@@ -466,6 +477,42 @@ __device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float
     return f;
 }
 
+// Per-wavefront state of the queue kernel; every member is uniform across the
+// wavefront.  pend[s] counts the queued queries of the chunk in slot s (-1: the
+// slot is free).
+struct WaveState {
+    uint32_t head, tail;   // FIFO entries [head, tail), modulo kQueue
+    int pend[kSlots];
+};
+
+// Walk the oldest min(64, queued) entries, put their results in their slots,
+// and store every chunk that has no queued query left.
+__device__ inline void walk_head(const EvalParams& P, WaveQueue& Q, WaveState& S, int w, int lane, int64_t off,
+                                 int64_t N, float* out) {
+    const uint32_t k = min(64u, S.tail - S.head);
+    lds_fence();
+    uint32_t slot = kSlots;   // no slot for idle lanes
+    if ((uint32_t)lane < k) {
+        const uint32_t pos = (S.head + lane) & (kQueue - 1);
+        const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.base[w][pos],
+                                  Q.mask[w][pos]);
+        const uint32_t t = Q.tag[w][pos];
+        slot = t >> 6;
+        Q.res[w][slot][t & 63] = r;
+    }
+    S.head += k;
+    lds_fence();
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        S.pend[s] -= __popcll(__ballot(slot == (uint32_t)s));
+        if (S.pend[s] == 0) {
+            const int64_t i = Q.start[w][s] + lane;
+            if (i < N) out[off + i] = Q.res[w][s][lane];
+            S.pend[s] = -1;
+        }
+    }
+}
+
 template <class Loader>
 __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t N, EvalParams P, float* out) {
     __shared__ WaveQueue Q;
@@ -473,9 +520,26 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
     const int lc = P.lc;
     const float sc = pow2neg(lc);
     const uint64_t below = (1ull << lane) - 1ull;
-    int qn = 0;   // entries on this wavefront's stack (uniform)
+    WaveState S;
+    S.head = S.tail = 0u;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) S.pend[s] = -1;
+    int cur = 0;   // slot of the next chunk (round-robin)
     const int64_t step = (int64_t)gridDim.x * kT;
-    for (int64_t c = (int64_t)blockIdx.x * kT + w * 64; c < N; c += step) {
+    int64_t c = (int64_t)blockIdx.x * kT + w * 64;
+    // One loop with a single walk site: walk the FIFO's head when it holds 64
+    // entries, when the next chunk's slot still waits for queued queries
+    // (they are the FIFO's oldest entries), or to drain it at the end.
+    for (;;) {
+        const bool more = c < N;
+        bool busy = false;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) busy |= (s == cur) && S.pend[s] > 0;
+        if (S.tail - S.head >= 64u || (more && busy) || (!more && S.tail != S.head)) {
+            walk_head(P, Q, S, w, lane, off, N, out);
+            continue;
+        }
+        if (!more) break;
         const int64_t i = c + lane;
         bool push = false;
         float f = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
@@ -491,36 +555,29 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
             f = tri_level(g0, g1, ux, uy, uz, sc);
         }
         const uint64_t bal = __ballot(push);
-        if (push) {
-            const int pos = qn + __popcll(bal & below);
-            Q.ux[w][pos] = ux;
-            Q.uy[w][pos] = uy;
-            Q.uz[w][pos] = uz;
-            Q.f[w][pos] = f;
-            Q.i[w][pos] = (uint32_t)i;
-            Q.base[w][pos] = lk.x;
-            Q.mask[w][pos] = lk.y;
-        } else if (i < N) {
-            out[off + i] = f;
+        if (bal == 0ull) {
+            if (i < N) out[off + i] = f;   // nothing queued: the chunk is complete
+        } else {
+            if (push) {
+                const uint32_t pos = (S.tail + __popcll(bal & below)) & (kQueue - 1);
+                Q.ux[w][pos] = ux;
+                Q.uy[w][pos] = uy;
+                Q.uz[w][pos] = uz;
+                Q.f[w][pos] = f;
+                Q.tag[w][pos] = ((uint32_t)cur << 6) | (uint32_t)lane;
+                Q.base[w][pos] = lk.x;
+                Q.mask[w][pos] = lk.y;
+            } else {
+                Q.res[w][cur][lane] = f;
+            }
+            S.tail += (uint32_t)__popcll(bal);
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s)
+                if (s == cur) S.pend[s] = __popcll(bal);
+            if (lane == 0) Q.start[w][cur] = c;
+            cur = (cur + 1) & (kSlots - 1);
         }
-        qn += __popcll(bal);
-        if (qn >= 64) {
-            lds_fence();
-            const int pos = qn - 64 + lane;
-            const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.base[w][pos],
-                                      Q.mask[w][pos]);
-            out[off + Q.i[w][pos]] = r;
-            qn -= 64;
-            lds_fence();
-        }
-    }
-    if (qn > 0) {
-        lds_fence();
-        if (lane < qn) {
-            const float r = fine_walk(P, Q.ux[w][lane], Q.uy[w][lane], Q.uz[w][lane], Q.f[w][lane], Q.base[w][lane],
-                                      Q.mask[w][lane]);
-            out[off + Q.i[w][lane]] = r;
-        }
+        c += step;
     }
 }
 
@@ -945,32 +1002,49 @@ void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box
     launch_eval(AosF64{pts}, s, N, P, f, st);
 }
 
-// queries [off[m], off[m+1]) belong to octree m; support semantics with the
-// octree's own box end (origin + width) and out_val for points outside
-void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
-                const float* z, float out_val, float* f, hipStream_t st) {
-    if (count <= 0 || off[count] <= 0) return;
-    std::vector<EvalParams> P(count);
-    std::vector<BatchItem> items;
-    constexpr int64_t kChunkQ = 16384;
+namespace {
+constexpr int64_t kChunkQ = 16384;   // queries per work item of the batched launch
+
+int64_t batch_item_count(int64_t count, const int64_t* off) {
+    int64_t n = 0;
+    for (int64_t m = 0; m < count; ++m) n += (off[m + 1] - off[m] + kChunkQ - 1) / kChunkQ;
+    return n;
+}
+// per-mesh parameters and the work items (mesh, begin, end) of a batch
+void batch_tables(its_svo const* svos, int64_t count, const int64_t* off, float out_val, EvalParams* P,
+                  BatchItem* items) {
+    int64_t k = 0;
     for (int64_t m = 0; m < count; ++m) {
         its_svo s = svos[m];
         const double end[3] = {s->origin[0] + s->width[0], s->origin[1] + s->width[1], s->origin[2] + s->width[2]};
         P[m] = make_params(s, end, ITS_EVAL_SUPPORT, out_val);
         for (int64_t b = off[m]; b < off[m + 1]; b += kChunkQ)
-            items.push_back(BatchItem{b, std::min(off[m + 1], b + kChunkQ), (int32_t)m, 0});
+            items[k++] = BatchItem{b, std::min(off[m + 1], b + kChunkQ), (int32_t)m, 0};
     }
-    DBuf<EvalParams> dP(count);
-    DBuf<BatchItem> dI((int64_t)items.size());
-    ITS_HIP(hipMemcpyAsync(dP.p, P.data(), count * sizeof(EvalParams), hipMemcpyHostToDevice, st));
-    ITS_HIP(hipMemcpyAsync(dI.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
-    const its_ctx ctx = svos[0]->ctx;
-    const unsigned grid = (unsigned)std::min<int64_t>((int64_t)items.size(),
-                                                      (int64_t)ctx->prop.multiProcessorCount *
-                                                          tune_get(ITS_TUNE_EVAL_BLOCKS_PER_CU));
-    hipLaunchKernelGGL(k_eval_batch, dim3(grid), dim3(kT), 0, st, (const EvalParams*)dP.p, (const BatchItem*)dI.p,
-                       (int64_t)items.size(), x, y, z, f);
+}
+void launch_batch(its_ctx ctx, const EvalParams* dP, const BatchItem* dI, int64_t n_items, const float* x,
+                  const float* y, const float* z, float* f, hipStream_t st) {
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        n_items, (int64_t)ctx->prop.multiProcessorCount * tune_get(ITS_TUNE_EVAL_BLOCKS_PER_CU));
+    hipLaunchKernelGGL(k_eval_batch, dim3(grid), dim3(kT), 0, st, dP, dI, n_items, x, y, z, f);
     ITS_LAUNCH_CHECK();
+}
+}  // namespace
+
+// queries [off[m], off[m+1]) belong to octree m; support semantics with the
+// octree's own box end (origin + width) and out_val for points outside
+void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
+                const float* z, float out_val, float* f, hipStream_t st) {
+    if (count <= 0 || off[count] <= 0) return;
+    const int64_t n_items = batch_item_count(count, off);
+    std::vector<EvalParams> P(count);
+    std::vector<BatchItem> items(n_items);
+    batch_tables(svos, count, off, out_val, P.data(), items.data());
+    DBuf<EvalParams> dP(count);
+    DBuf<BatchItem> dI(n_items);
+    ITS_HIP(hipMemcpyAsync(dP.p, P.data(), count * sizeof(EvalParams), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(dI.p, items.data(), n_items * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    launch_batch(svos[0]->ctx, dP.p, dI.p, n_items, x, y, z, f, st);
     ITS_HIP(hipStreamSynchronize(st));
 }
 
