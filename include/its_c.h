@@ -310,6 +310,25 @@ int its_batch_fields(its_batch b);
 int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const float* d_z, const int64_t* offsets,
                           float out_val, float* d_f, double* seconds);
 
+/* its_batch_eval_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-mesh parameters and work items (a mesh's queries cut into chunks of
+ * ITS_NOALLOC_CHUNK) live in a fixed device array of the library, one per
+ * device, about 1.9 MB, set up by its_ctx_create and held until the process
+ * ends; host-side staging is static too.  Limits: at most
+ * ITS_NOALLOC_MAX_MESHES meshes and ITS_NOALLOC_MAX_ITEMS chunks per call,
+ * else ITS_E_RANGE before anything is launched.  Results are bit-identical to
+ * its_batch_eval_device.  Calls share the array of their device: calls from
+ * one process are serialised by the library and each waits for its launch;
+ * calls from several processes on one GPU each have their own array.  The
+ * fields must already exist (its_batch_fields). */
+#define ITS_NOALLOC_MAX_MESHES 1024
+#define ITS_NOALLOC_MAX_ITEMS 65536
+#define ITS_NOALLOC_CHUNK 16384
+int its_batch_eval_device_noalloc(its_batch b, const float* d_x, const float* d_y, const float* d_z,
+                                  const int64_t* offsets, float out_val, float* d_f, double* seconds);
+
 /* ---- tunables (process-wide; set by the autotuner, its_amd/autotune.py) ----
  * ITS_TUNE_DENSE_CELLS: the evaluation field collapses every level whose
  *   cell count is at most this into one dense block grid (default 32768).

@@ -20,6 +20,7 @@ ITS_MAX_LEVELS = 12
 EVAL_SUPPORT, EVAL_DENSE, EVAL_RAW = 0, 1, 2
 TUNE_DENSE_CELLS, TUNE_EVAL_BLOCKS_PER_CU, TUNE_COARSE_VERTS, TUNE_EVAL_ILP, TUNE_EVAL_QUEUE = 0, 1, 2, 3, 4
 TUNE_BATCH_MAX_VERTS = 5
+NOALLOC_MAX_MESHES, NOALLOC_MAX_ITEMS, NOALLOC_CHUNK = 1024, 65536, 16384
 E_INVALID, E_HIP, E_NOMEM, E_STATE, E_RANGE = -1, -2, -3, -4, -5
 
 
@@ -148,6 +149,8 @@ SIGNATURES = {
     "its_batch_fields": [vp],
     "its_batch_eval_device": [vp, vp, vp, vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_float,
                               vp, C.POINTER(C.c_double)],
+    "its_batch_eval_device_noalloc": [vp, vp, vp, vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                      C.c_float, vp, C.POINTER(C.c_double)],
     "its_mlp_create": [vp, C.c_uint64, C.POINTER(vp)],
     "its_mlp_destroy": [vp],
     "its_mlp_set_params": [vp, _f32p],
@@ -586,6 +589,14 @@ class Batch:
         sec = C.c_double(0)
         check(lib().its_batch_eval_device(self._h, vp(dx), vp(dy), vp(dz), np.ascontiguousarray(offsets, np.int64),
                                           float(out_val), vp(d_out), C.byref(sec)))
+        return sec.value
+
+    def eval_device_noalloc(self, dx, dy, dz, offsets, d_out, out_val=0.0):
+        """eval_device with the library's fixed scratch array: no allocation."""
+        sec = C.c_double(0)
+        check(lib().its_batch_eval_device_noalloc(self._h, vp(dx), vp(dy), vp(dz),
+                                                  np.ascontiguousarray(offsets, np.int64), float(out_val), vp(d_out),
+                                                  C.byref(sec)))
         return sec.value
 
     def close(self):

@@ -94,6 +94,8 @@ int its_ctx_create(int device, its_ctx* out) {
     ITS_HIP(hipGetDeviceCount(&n));
     ITS_REQUIRE(device >= 0 && device < n && device < 64, ITS_E_INVALID, "no such device");
     DeviceGuard g(device);
+    // device address of the allocation-free evaluation's scratch array
+    eval_scratch_init(device);
     auto* c = new its_ctx_s();
     c->device = device;
     ITS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -829,6 +831,26 @@ int its_batch_eval_device(its_batch b, const float* d_x, const float* d_y, const
     ITS_API_END
 }
 
+int its_batch_eval_device_noalloc(its_batch b, const float* d_x, const float* d_y, const float* d_z,
+                                  const int64_t* offsets, float out_val, float* d_f, double* seconds) {
+    ITS_API_BEGIN
+    check_batch_eval(b, d_x, d_y, d_z, offsets, d_f);
+    const int64_t count = batch_count(b);
+    ITS_REQUIRE(count <= ITS_NOALLOC_MAX_MESHES, ITS_E_RANGE,
+                "batch holds more meshes than the allocation-free evaluation takes (ITS_NOALLOC_MAX_MESHES)");
+    its_ctx ctx = batch_ctx(b);
+    DeviceGuard g(ctx->device);
+    its_svo v[ITS_NOALLOC_MAX_MESHES];
+    for (int64_t m = 0; m < count; ++m) v[m] = batch_svo(b, m);
+    ITS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    eval_batch_noalloc(v, count, offsets, d_x, d_y, d_z, out_val, d_f, ctx->stream);
+    ITS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ITS_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    ITS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (seconds) *seconds = ms * 1e-3;
+    ITS_API_END
+}
 
 // ---- learned warm start
 static void check_mlp(its_mlp net) { ITS_REQUIRE(net, ITS_E_INVALID, "null network"); }

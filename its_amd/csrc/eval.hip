@@ -37,6 +37,7 @@
 // and RAW modes).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -1003,7 +1004,7 @@ void eval_points_aos64(its_svo s, const double* pts, int64_t N, const double box
 }
 
 namespace {
-constexpr int64_t kChunkQ = 16384;   // queries per work item of the batched launch
+constexpr int64_t kChunkQ = ITS_NOALLOC_CHUNK;   // queries per work item of the batched launch
 
 int64_t batch_item_count(int64_t count, const int64_t* off) {
     int64_t n = 0;
@@ -1045,6 +1046,58 @@ void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const fl
     ITS_HIP(hipMemcpyAsync(dP.p, P.data(), count * sizeof(EvalParams), hipMemcpyHostToDevice, st));
     ITS_HIP(hipMemcpyAsync(dI.p, items.data(), n_items * sizeof(BatchItem), hipMemcpyHostToDevice, st));
     launch_batch(svos[0]->ctx, dP.p, dI.p, n_items, x, y, z, f, st);
+    ITS_HIP(hipStreamSynchronize(st));
+}
+
+// ---- batched evaluation that allocates nothing
+// The per-mesh parameters and the work items go to a file-scope device array
+// instead of stream-ordered buffers.  Every device holds its own copy of the
+// array, at its own address, until the process ends.  Host code never uses
+// the array's name as an address: eval_scratch_init takes the address on each
+// device with hipGetSymbolAddress, and the launches use that.  The host side
+// builds the tables in a static staging area of the same layout, so a call
+// allocates neither device nor host memory.  Calls share the one array per
+// device (and the staging area): the mutex orders calls of this process, and
+// each call waits for its launch before it returns.
+struct EvalScratch {
+    EvalParams P[kNoAllocMeshes];
+    BatchItem I[kNoAllocItems];
+};
+static __device__ __align__(256) EvalScratch scratch;
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+EvalScratch* g_scratch_dev[kMaxDevices] = {};   // address of each device's array
+EvalScratch g_scratch_host;                     // staging, guarded by g_scratch_mu
+}  // namespace
+
+void eval_scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return;
+    void* p = nullptr;
+    ITS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ITS_REQUIRE(p != nullptr, ITS_E_HIP, "no device address for the evaluation scratch");
+    g_scratch_dev[device] = (EvalScratch*)p;
+}
+
+void eval_batch_noalloc(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
+                        const float* z, float out_val, float* f, hipStream_t st) {
+    ITS_REQUIRE(count <= kNoAllocMeshes, ITS_E_RANGE,
+                "batch holds more meshes than the allocation-free evaluation takes (ITS_NOALLOC_MAX_MESHES)");
+    const int64_t n_items = batch_item_count(count, off);
+    ITS_REQUIRE(n_items <= kNoAllocItems, ITS_E_RANGE,
+                "too many queries for the allocation-free evaluation (ITS_NOALLOC_MAX_ITEMS chunks of "
+                "ITS_NOALLOC_CHUNK queries)");
+    if (count <= 0 || off[count] <= 0) return;
+    const its_ctx ctx = svos[0]->ctx;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    EvalScratch* d = g_scratch_dev[ctx->device];
+    ITS_REQUIRE(d != nullptr, ITS_E_STATE, "evaluation scratch not set up on this device");
+    batch_tables(svos, count, off, out_val, g_scratch_host.P, g_scratch_host.I);
+    ITS_HIP(hipMemcpyAsync(d->P, g_scratch_host.P, count * sizeof(EvalParams), hipMemcpyHostToDevice, st));
+    ITS_HIP(hipMemcpyAsync(d->I, g_scratch_host.I, n_items * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    launch_batch(ctx, d->P, d->I, n_items, x, y, z, f, st);
     ITS_HIP(hipStreamSynchronize(st));
 }
 

@@ -138,6 +138,12 @@ void nearest_normals(hipStream_t st, const double* d_V, const int32_t* d_F, cons
                      float4* d_nrm);
 void eval_batch(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
                 const float* z, float out_val, float* f, hipStream_t st);
+// the same launch with its tables in the file-scope scratch array of eval.hip
+constexpr int64_t kNoAllocMeshes = ITS_NOALLOC_MAX_MESHES;
+constexpr int64_t kNoAllocItems = ITS_NOALLOC_MAX_ITEMS;
+void eval_scratch_init(int device);   // once per device, with that device current
+void eval_batch_noalloc(its_svo const* svos, int64_t count, const int64_t* off, const float* x, const float* y,
+                        const float* z, float out_val, float* f, hipStream_t st);
 // shell.hip
 // level-0 node edges with owners in the reference's fineNodeEdgeArray order;
 // returns the count
