@@ -123,6 +123,16 @@ def main():
     torch.cuda.set_stream(stream)
     ctx = _capi.Context(0)
     ctx.set_stream(stream.cuda_stream)
+    # launch shapes recorded by the autotuner for this depth
+    from its_amd import autotune
+    tuned_depth = autotune.apply(R.bit_length() - 1)
+    tuning = {"table": "its_amd/data/autotune.json", "depth": tuned_depth,
+              **{k: _capi.tune_get(v) for k, v in (("dense_cells", _capi.TUNE_DENSE_CELLS),
+                                                   ("eval_blocks_per_cu", _capi.TUNE_EVAL_BLOCKS_PER_CU),
+                                                   ("eval_ilp", _capi.TUNE_EVAL_ILP),
+                                                   ("eval_queue", _capi.TUNE_EVAL_QUEUE),
+                                                   ("coarse_verts", _capi.TUNE_COARSE_VERTS))}}
+    log(f"[bench] tuning {tuning}")
 
     log("[bench] mesh")
     V, F, origin, end, width = make_mesh()
@@ -356,6 +366,7 @@ def main():
             "unknowns": svo.num_verts,
             "nodes": svo.num_nodes,
         },
+        "tuning": tuning,
         "eval_kernel_ms": {"mean": avg_ms, "buffer_sets": K_SETS, "mean_one_set": avg_ms_1,
                            "note": "mean is over launches that take the query/output sets in turn (640 MB, more "
                                    "than the 256 MB Infinity Cache); mean_one_set repeats set 0",
