@@ -232,6 +232,25 @@ def intersections(svo: SVO, V, F):
     return out[:n], int(ne[0])
 
 
+def shell_iso(svo: SVO, lam, V, F):
+    """(inner, outer) iso values: min and max of the latent values at the
+    mesh vertices and the intersection points (cpLatentBSplineValue,
+    src/ThinShells.cpp:655-755, 743-744)."""
+    Q, _ = intersections(svo, V, F)
+    f, rc = svo.eval_latent(lam, np.concatenate([_f64(V), Q]))
+    if rc != 0:
+        raise ValueError("a latent point's fine voxel is not an octree node")
+    return float(f.min()), float(f.max())
+
+
+def classify(f, inner, outer):
+    """{-1, 0, 1} classes of query values (multiPointQuery,
+    src/ThinShells.cpp:1412-1417): 1 at or above the outer iso value, -1 at
+    or below the inner one, 0 in between (a NaN compares false: 0)."""
+    f = np.asarray(f, np.float64)
+    return np.where(f >= outer, 1, np.where(f <= inner, -1, 0)).astype(np.int8)
+
+
 def fine_edges(svo: SVO):
     """Level-0 node edges (fineNodeEdgeArray, src/cuAcc/SVO.cu:965-1110):
     (n, 2, 3) end points and the owner node of each, in the reference's order."""

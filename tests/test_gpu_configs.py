@@ -106,3 +106,64 @@ def test_trained_warm_start_held_out(ctx):
     assert iw.mean() <= ic.mean()
     net.close()
     untrained.close()
+
+
+def test_precision_blob_512_against_oracle(ctx):
+    """The precision workload at full size: the blob union of 32 spheres
+    (seed 42) remeshed to about 1.5 M triangles, R = 512 (ten levels).
+    - octree: every node and vertex field bit for bit against the oracle's
+      build (SparseVoxelOctree::createOctree, src/cuAcc/SVO.cu:674-1113);
+    - fit: Eigen's stopping quantity ||A^T(b - A lambda)|| / ||A^T b||
+      recomputed in binary64 with the oracle's assembled matrix
+      (cpCoefficients, src/ThinShells.cpp:511-617) is below 1e-6;
+    - evaluation: f at 20,000 octree vertices and 20,000 uniform points,
+      binary32 on the GPU, against the oracle's binary64 support evaluation
+      with the same coefficients (mt_cpuTest_2, src/ThinShells.cpp:1249-1294),
+      within 1e-5."""
+    import oracle as O
+    import precision_report as P
+
+    from its_amd import _capi, meshes
+
+    R = 512
+    (V, F), mres = P.blob_mesh(42, 1.5e6)
+    V = _capi.model_to_unit_cube(V)
+    origin, end, width = _capi.uniform_bbox(V)
+    tris = meshes.triangles(V, F)
+    print(f"[blob512] {len(F)} triangles (grid {mres}^3); GPU octree, fit and oracle octree", flush=True)
+    assert 1.2e6 <= len(F) <= 1.8e6
+    g = _capi.Octree(ctx, tris, origin, width, R)
+    b = g.sdf_at_vertices(V, F)
+    lam, st = g.lscg(b, tol=1e-6)
+    ref = O.SVO(tris, origin, width, R)
+    assert g.depth == ref.depth == 10
+    assert g.num_nodes == ref.num_nodes and g.num_verts == ref.num_verts
+    nd = g.nodes()
+    for k, r in [("morton", ref.morton), ("is_leaf", ref.is_leaf), ("parent", ref.parent), ("childs", ref.childs),
+                 ("neighbors", ref.neighbors)]:
+        np.testing.assert_array_equal(nd[k], r, err_msg=k)
+    np.testing.assert_array_equal(nd["origin"].view(np.uint64), ref.node_origin.view(np.uint64))
+    np.testing.assert_array_equal(nd["width"].view(np.uint64), ref.node_width.view(np.uint64))
+    xyz, owner = g.verts()
+    np.testing.assert_array_equal(owner, ref.vert_owner)
+    np.testing.assert_array_equal(xyz.view(np.uint64), ref.vert_xyz.view(np.uint64))
+    print(f"[blob512] octree bit-exact: {g.num_nodes} nodes, {g.num_verts} vertices; assembling the oracle matrix",
+          flush=True)
+    A = O.Matrix(ref)
+    r = b - A.mul(lam)
+    g_rel = np.linalg.norm(A.tmul(r)) / np.linalg.norm(A.tmul(b))
+    print(f"[blob512] fit: {st['iterations']} iterations, normal_rel {st['normal_rel']:.3e} (library), "
+          f"{g_rel:.3e} (oracle matrix); residual_rel {np.linalg.norm(r) / np.linalg.norm(b):.3e}", flush=True)
+    assert st["converged"] and g_rel < 1e-6
+    del A
+    rng = np.random.default_rng(5)
+    sel = rng.choice(g.num_verts, 20_000, replace=False)
+    pts = np.concatenate([xyz[sel], rng.uniform(origin - 0.05, end + 0.05, size=(20_000, 3))])
+    p32 = pts.astype(np.float32)
+    f = g.eval(p32, end, _capi.EVAL_SUPPORT, 0.0)
+    f_ref = ref.eval_support(lam, p32.astype(np.float64), end, 0.0)
+    err = np.abs(f - f_ref)
+    print(f"[blob512] f at 20,000 vertices: max err {err[:20_000].max():.3e}; at 20,000 uniform points: "
+          f"{err[20_000:].max():.3e}", flush=True)
+    assert err.max() <= 1e-5
+    g.close()

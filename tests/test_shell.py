@@ -75,3 +75,47 @@ def test_gpu_iso_values(ctx, name, R):
     assert abs(inner - f.min()) <= 1e-5 and abs(outer - f.max()) <= 1e-5
     assert inner < 0 < outer
     g.close()
+
+
+def test_oracle_classify_rule():
+    """The class rule on its boundaries: a value equal to the outer iso value
+    is 1, equal to the inner one -1, NaN 0 (ThinShells.cpp:1412-1417)."""
+    inner, outer = -0.25, 0.5
+    f = np.array([-1.0, -0.25, -0.2499999, 0.0, 0.4999999, 0.5, 2.0, np.nan, np.inf, -np.inf])
+    assert O.classify(f, inner, outer).tolist() == [-1, -1, 0, 0, 0, 1, 1, 0, 1, -1]
+    # a band of zero width (inner == outer) sends every value to one side
+    assert O.classify(np.array([0.1, 0.2, 0.3]), 0.2, 0.2).tolist() == [-1, 1, 1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,R", [("icosphere", 32), ("bunny", 64), ("knot_small", 32)])
+def test_gpu_classes_match_oracle(ctx, name, R):
+    """multiPointQuery's classes (ThinShells.cpp:1183-1420): the library's
+    iso values, binary32 support evaluation and its_classify against the
+    oracle's iso values, binary64 support evaluation and class rule, with the
+    same coefficients.  Classes may differ only where a value lies within the
+    evaluation tolerance of an iso value."""
+    from its_amd import _capi
+    F = fitted(name, R)
+    c = F.case
+    g = _capi.Octree(ctx, c.tris, c.origin, c.width, R)
+    g.set_lambda(F.lam)
+    inner, outer, _ = g.shell_iso(c.V, c.F)
+    inner_r, outer_r = O.shell_iso(F.svo, F.lam, c.V, c.F)
+    assert abs(inner - inner_r) <= 1e-5 and abs(outer - outer_r) <= 1e-5
+    rng = np.random.default_rng(17)
+    Q = rng.uniform(c.origin - 0.05, c.end + 0.05, size=(200_000, 3))
+    # points near the surface too, where the classes change
+    Q = np.concatenate([Q, c.V + rng.normal(scale=0.5 * c.width[0] / R, size=c.V.shape)])
+    fq = g.eval(Q.astype(np.float32), c.end, _capi.EVAL_SUPPORT, np.float32(outer))
+    cls = _capi.classify(fq, inner, outer)
+    f_r = F.svo.eval_support(F.lam, Q.astype(np.float32).astype(np.float64), c.end, outer_r)
+    cls_r = O.classify(f_r, inner_r, outer_r)
+    near = (np.abs(f_r - inner_r) <= 2e-5) | (np.abs(f_r - outer_r) <= 2e-5)
+    bad = (cls != cls_r) & ~near
+    print(f"[classes] {name} R={R}: {len(Q)} queries, classes -1/0/1 = "
+          f"{[(int((cls_r == k).sum())) for k in (-1, 0, 1)]}, {int((cls != cls_r).sum())} differ, all near an iso value",
+          flush=True)
+    assert not bad.any()
+    assert (cls_r == 0).sum() > 100 and (cls_r == -1).sum() > 100 and (cls_r == 1).sum() > 100
+    g.close()
