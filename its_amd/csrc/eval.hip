@@ -605,7 +605,10 @@ __global__ void k_dense_level(int D, int zrun, const double* __restrict__ gc, in
         const int Z1 = min(Z0 + zrun, D);
         const int x0 = X >> 1, x1 = (X + 1) >> 1, y0 = Y >> 1, y1 = (Y + 1) >> 1;
         const bool two_x = X & 1, two_y = Y & 1;
-        const double nxy = (double)((two_x ? 2 : 1) * (two_y ? 2 : 1));
+        // 1 / (number of coarse points averaged): 1, 1/2 or 1/4 in the plane,
+        // half that between planes; powers of two, so the products are the
+        // quotients exactly
+        const double rxy = (two_x ? 0.5 : 1.0) * (two_y ? 0.5 : 1.0);
         const uint32_t key0 = (uint32_t)(((int64_t)X * D + Y) * D);
         const uint32_t wlo = (key0 + (uint32_t)Z0) >> 5;
         const RankWord w0 = lat[wlo], w1 = lat[(key0 + (uint32_t)Z1 - 1u) >> 5];
@@ -627,10 +630,10 @@ __global__ void k_dense_level(int D, int zrun, const double* __restrict__ gc, in
             if (gc) {
                 if (Z & 1) {
                     const double nxt = plane((Z + 1) >> 1);
-                    v = (cur + nxt) / (2.0 * nxy);
+                    v = (cur + nxt) * (0.5 * rxy);
                     cur = nxt;
                 } else {
-                    v = cur / nxy;
+                    v = cur * rxy;
                 }
             }
             const uint32_t key = key0 + (uint32_t)Z;
@@ -642,47 +645,66 @@ __global__ void k_dense_level(int D, int zrun, const double* __restrict__ gc, in
     }
 }
 // The small top levels of the cascade (at most kTopD points per axis) in one
-// workgroup, level after level with a barrier between them: one launch in
-// place of one per level.  Same arithmetic per point as k_dense_level.
+// workgroup: one launch in place of one per level.  Every point's own
+// coefficient (0 where the point is not a vertex) is fetched first, all
+// levels at once, into LDS; the levels then follow each other in LDS with a
+// barrier between them, and each level is also written out.  Same
+// arithmetic per point as k_dense_level.
 constexpr int kTopD = 17;
 constexpr int kTopThreads = 1024;
+constexpr int kTopPoints = 8 + 27 + 125 + 729 + 4913;   // D = 2, 3, 5, 9, 17: the most the top levels hold
 struct TopLevels {
     int n;                          // levels, coarsest first
     int D[kMaxLevels];              // points per axis
+    int base[kMaxLevels + 1];       // first LDS slot of each level
     const RankWord* lat[kMaxLevels];
     int64_t voff[kMaxLevels];
     double* out[kMaxLevels];
 };
 __global__ __launch_bounds__(kTopThreads) void k_dense_top(TopLevels T, const double* __restrict__ lambda) {
+    __shared__ double sv[kTopPoints];
+    // own coefficients of every point of every top level
     for (int l = 0; l < T.n; ++l) {
-        const int D = T.D[l], Dc = l ? T.D[l - 1] : 0;
-        const double* gc = l ? T.out[l - 1] : nullptr;
+        const int D = T.D[l], n = D * D * D;
+        for (int t = threadIdx.x; t < n; t += kTopThreads) {
+            const int X = t % D, Y = (t / D) % D, Z = t / (D * D);
+            const uint32_t r = bit_find(T.lat[l], (uint32_t)(((int64_t)X * D + Y) * D + Z));
+            // below the coarsest level a point that is not a vertex adds -0.0,
+            // which leaves every sum as it is (a -0.0 sum included); the
+            // coarsest level's values are 0.0 + lambda, as in k_dense_level
+            const double lam = r != ITS_NONE ? lambda[T.voff[l] + r] : (l ? -0.0 : 0.0);
+            sv[T.base[l] + t] = l ? lam : 0.0 + lam;
+        }
+    }
+    __syncthreads();
+    for (int l = 1; l < T.n; ++l) {
+        const int D = T.D[l], Dc = T.D[l - 1];
+        const double* gc = sv + T.base[l - 1];
         const int n = D * D * D;
         for (int t = threadIdx.x; t < n; t += kTopThreads) {
             const int X = t % D, Y = (t / D) % D, Z = t / (D * D);
-            double v = 0.0;
-            if (gc) {
-                const int xs[2] = {X >> 1, (X + 1) >> 1}, ys[2] = {Y >> 1, (Y + 1) >> 1}, zs[2] = {Z >> 1, (Z + 1) >> 1};
-                const bool tx = X & 1, ty = Y & 1, tz = Z & 1;
-                const double nxy = (double)((tx ? 2 : 1) * (ty ? 2 : 1));
-                auto plane = [&](int p) {
-                    const double* r0 = gc + ((int64_t)p * Dc + ys[0]) * Dc;
-                    double s = r0[xs[0]];
-                    if (tx) s += r0[xs[1]];
-                    if (ty) {
-                        const double* r1 = gc + ((int64_t)p * Dc + ys[1]) * Dc;
-                        s += r1[xs[0]];
-                        if (tx) s += r1[xs[1]];
-                    }
-                    return s;
-                };
-                v = tz ? (plane(zs[0]) + plane(zs[1])) / (2.0 * nxy) : plane(zs[0]) / nxy;
-            }
-            const uint32_t r = bit_find(T.lat[l], (uint32_t)(((int64_t)X * D + Y) * D + Z));
-            if (r != ITS_NONE) v += lambda[T.voff[l] + r];
-            T.out[l][t] = v;
+            const int xs[2] = {X >> 1, (X + 1) >> 1}, ys[2] = {Y >> 1, (Y + 1) >> 1}, zs[2] = {Z >> 1, (Z + 1) >> 1};
+            const bool tx = X & 1, ty = Y & 1, tz = Z & 1;
+            const double rxy = (tx ? 0.5 : 1.0) * (ty ? 0.5 : 1.0);
+            auto plane = [&](int p) {
+                const double* r0 = gc + (p * Dc + ys[0]) * Dc;
+                double s = r0[xs[0]];
+                if (tx) s += r0[xs[1]];
+                if (ty) {
+                    const double* r1 = gc + (p * Dc + ys[1]) * Dc;
+                    s += r1[xs[0]];
+                    if (tx) s += r1[xs[1]];
+                }
+                return s;
+            };
+            const double v = tz ? (plane(zs[0]) + plane(zs[1])) * (0.5 * rxy) : plane(zs[0]) * rxy;
+            sv[T.base[l] + t] = v + sv[T.base[l] + t];
         }
         __syncthreads();
+    }
+    for (int l = 0; l < T.n; ++l) {
+        const int n = T.D[l] * T.D[l] * T.D[l];
+        for (int t = threadIdx.x; t < n; t += kTopThreads) T.out[l][t] = sv[T.base[l] + t];
     }
 }
 
@@ -939,6 +961,7 @@ const double* cascade_above(its_svo s, int stop, hipStream_t st) {
     TopLevels T{};
     for (; d > stop && s->lv[d].R + 1 <= kTopD; --d, k ^= 1) {
         const int D = s->lv[d].R + 1;
+        T.base[T.n + 1] = T.base[T.n] + D * D * D;
         T.D[T.n] = D;
         T.lat[T.n] = s->lv[d].lattice.p;
         T.voff[T.n] = s->esum_verts[d];
@@ -948,6 +971,7 @@ const double* cascade_above(its_svo s, int stop, hipStream_t st) {
         Dprev = D;
     }
     if (T.n) {
+        ITS_REQUIRE(T.base[T.n] <= kTopPoints, ITS_E_STATE, "top cascade levels exceed their LDS space");
         hipLaunchKernelGGL(k_dense_top, dim3(1), dim3(kTopThreads), 0, st, T, (const double*)s->lambda64.p);
         ITS_LAUNCH_CHECK();
     }

@@ -92,10 +92,9 @@ struct its_svo_s {
     // marching-cubes lattice, cube cases, per-step vertex counts and soup
     its::DBuf<double> casc[2];
     its::DBuf<float> mc_lat, mc_soup;
-    its::DBuf<uint8_t> mc_case;
     its::DBuf<uint64_t> mc_step;
-    its::DBuf<unsigned long long> mc_valid;   // [0] voxels with triangles and listed steps, packed; [1] vertices
-    its::DBuf<uint32_t> mc_list;              // steps with vertices
+    its::DBuf<unsigned long long> mc_valid;   // [0] voxels with triangles; [1] vertices
+    its::DBuf<uint64_t> mc_list;              // voxels with triangles: step, lane, case, offset in the step
 
     std::unique_ptr<its::Fit, void (*)(its::Fit*)> fit{nullptr, nullptr};
 };

@@ -8,11 +8,12 @@
 // force sum over all n basis functions, batched over 1024 streams.  Here f is
 // evaluated once per lattice point ((res+1)^3 points), by the dense cascade on
 // the octree's own lattice or by the octree field walk elsewhere.  A count
-// pass writes every voxel's cube case and the vertex count of every step of
-// 64 voxels along x, and lists the steps that have vertices; an exclusive
-// scan of the step counts gives each step its offset; an emit pass, one wave
-// per listed step, places each voxel's triangles with a wave prefix sum.  The
-// soup order is the reference's: voxels x-fastest, triangles in table order.
+// pass writes the vertex count of every step of 64 voxels along x and lists
+// the voxels that have triangles, each with its case and its offset within
+// its step; an exclusive scan of the step counts gives each step its offset;
+// an emit pass, one thread per listed voxel, writes the voxel's triangles.
+// The soup order is the reference's: voxels x-fastest, triangles in table
+// order.
 // The 1e-6 snaps of vertexLerp (MarchingCubes.cu:72-90) are kept, and so is
 // the lerp fma(t, v1, fma(-t, v0, v0)).
 #include <algorithm>
@@ -44,8 +45,8 @@ __constant__ unsigned char c_cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 // The count pass gives each wave one 64-voxel column chunk over J rows of
 // one z-slab: walking +y, the lattice rows at j+1 become those at j, so a
 // step loads two new rows (x and x+1 for the planes k and k+1), prefetched
-// one row ahead.  It writes each voxel's cube case and each step's vertex
-// count; a scan of the step counts gives each step its place in the soup.
+// one row ahead.  It writes each step's vertex count; a scan of the step
+// counts gives each step its place in the soup.
 struct McGrid {
     int rx, ry, rz;
     int64_t lx, ly;   // lattice points per axis in x, y (res + 1)
@@ -65,15 +66,29 @@ __device__ inline unsigned case8(const float (&v)[8], float iso) {
     return c;
 }
 
-// pass 1: cube cases, vertices per step, voxels with triangles, and the
-// list of steps with vertices.  One 64-bit counter holds both totals: listed
-// steps in the low kStepBits bits, voxels with triangles above them, so a
-// wave makes one atomic.  The list order depends on timing; each step's place
-// in the soup comes from the scan of step_verts, so the output does not.
-__global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, McGrid G, uint8_t* __restrict__ cases,
+// pass 1: vertices per step and the list of voxels with triangles.  Per row
+// step a wave forms the 64 cube cases; a voxel has triangles exactly when its
+// case is neither 0 nor 255 (every other case of the table has some), so
+// only rows with such a voxel look up vertex counts and sum them.  The cases
+// wait in LDS while the wave walks its rows; the wave then reserves one list
+// slot per voxel with triangles (one atomic) and, for the rows that have
+// them, forms each voxel's offset within its step (a wave prefix sum) and
+// writes its entry: step (kStepBits bits), lane (6), case (8), offset within
+// the step (10).  The list order depends on timing; each voxel's place in
+// the soup is its step's place (the scan of step_verts) plus its offset, so
+// the output does not.
+constexpr int kRows = 32;   // rows per count wave
+constexpr int kWaves = kT / 64;
+__device__ inline uint64_t pack_entry(int64_t s, int lane, unsigned c, unsigned off) {
+    return (uint64_t)s | ((uint64_t)lane << kStepBits) | ((uint64_t)c << (kStepBits + 6)) |
+           ((uint64_t)off << (kStepBits + 14));
+}
+__device__ inline bool has_triangles(unsigned c) { return c != 0u && c != 255u; }
+__global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, McGrid G,
                                                  uint64_t* __restrict__ step_verts, unsigned long long* counts,
-                                                 uint32_t* __restrict__ steps) {
-    const int lane = threadIdx.x & 63;
+                                                 uint64_t* __restrict__ entries) {
+    __shared__ uint8_t s_case[kWaves][kRows][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
     if (w >= G.nwaves) return;   // whole waves only
     const int ch = (int)(w % G.nch);
@@ -90,30 +105,43 @@ __global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, 
     float na0 = lat[p0 + (j0 + 1) * G.lx], na1 = lat[p0 + (j0 + 1) * G.lx + 1];
     float nb0 = lat[p1 + (j0 + 1) * G.lx], nb1 = lat[p1 + (j0 + 1) * G.lx + 1];
     unsigned nval = 0;
-    uint32_t mask = 0;   // bit j - j0: step (k, j, ch) has vertices (J <= 32)
+    uint32_t rows = 0;   // bit j - j0: row j has a voxel with triangles
     for (int j = j0; j < j1; ++j) {
         const int64_t jn = min(j + 2, G.ry) * G.lx;
         const float f0 = lat[p0 + jn], f1 = lat[p0 + jn + 1], g0 = lat[p1 + jn], g1 = lat[p1 + jn + 1];
         // the reference's corner order: (x,y,z) = (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1)
         const float v[8] = {a0, na0, na1, a1, b0, nb0, nb1, b1};
-        const unsigned c = case8(v, G.iso);
-        const unsigned nv = in ? (unsigned)c_nv[c] : 0u;
-        if (in) cases[((int64_t)k * G.ry + j) * G.rx + i] = (uint8_t)c;
-        const unsigned sum = wave_sum(nv);
-        nval += __popcll(__ballot(nv != 0));
+        const unsigned c = in ? case8(v, G.iso) : 0u;
+        s_case[wv][j - j0][lane] = (uint8_t)c;
+        const uint64_t bal = __ballot(has_triangles(c));
+        unsigned sum = 0;
+        if (bal) {
+            sum = wave_sum(has_triangles(c) ? (unsigned)c_nv[c] : 0u);
+            nval += __popcll(bal);
+            rows |= 1u << (j - j0);
+        }
         if (lane == 0) step_verts[((int64_t)k * G.ry + j) * G.nch + ch] = sum;
-        if (sum) mask |= 1u << (j - j0);
         a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
         na0 = f0; na1 = f1; nb0 = g0; nb1 = g1;
     }
-    if (!mask) return;
+    if (!nval) return;
     unsigned long long slot = 0;
-    if (lane == 0)
-        slot = atomicAdd(counts, ((unsigned long long)nval << kStepBits) | (unsigned long long)__popc(mask)) &
-               kStepMask;
+    if (lane == 0) slot = atomicAdd(counts, (unsigned long long)nval);
     slot = __shfl(slot, 0, 64);
-    if (lane < 32 && ((mask >> lane) & 1u))
-        steps[slot + __popc(mask & ((1u << lane) - 1u))] = (uint32_t)(((int64_t)k * G.ry + j0 + lane) * G.nch + ch);
+    const uint64_t below = (1ull << lane) - 1ull;
+    // every lane reads back only the cases it wrote itself
+    while (rows) {
+        const int r = __builtin_ctz(rows);
+        rows &= rows - 1u;
+        const unsigned c = s_case[wv][r][lane];
+        const bool act = has_triangles(c);
+        const unsigned nv = act ? (unsigned)c_nv[c] : 0u;
+        const unsigned incl = wave_incl_scan(nv);
+        const uint64_t bal = __ballot(act);
+        if (act) entries[slot + __popcll(bal & below)] = pack_entry(((int64_t)k * G.ry + j0 + r) * G.nch + ch, lane, c,
+                                                                    incl - nv);
+        slot += __popcll(bal);
+    }
 }
 
 struct P3 {
@@ -146,24 +174,28 @@ __device__ inline P3 corner_pos(const P3& vp, double vx, double vy, double vz, i
               vp.z + (((kCornerZ >> q) & 1) ? vz : 0.0)};
 }
 
-// pass 2: one wave per listed step (64 voxels of a row).  A wave prefix sum
-// over the 64 voxels places their triangles after those of all earlier
-// voxels (step_off: exclusive scan of the step counts).
-__device__ inline void emit_step(const float* __restrict__ lat, const McGrid& G, double ox, double oy, double oz,
-                                 double vx, double vy, double vz, const uint8_t* __restrict__ cases, uint64_t at,
-                                 int64_t s, uint64_t max_verts, float* __restrict__ out) {
-    const int lane = threadIdx.x & 63;
+// pass 2: one thread per voxel with triangles.  Its triangles start at its
+// step's place in the soup (step_off: exclusive scan of the step counts)
+// plus its offset within the step.
+__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
+                                                double oz, double vx, double vy, double vz,
+                                                const uint64_t* __restrict__ step_off,
+                                                const uint64_t* __restrict__ entries,
+                                                const unsigned long long* __restrict__ counts, uint64_t max_verts,
+                                                float* __restrict__ out) {
+    const int64_t nval = (int64_t)counts[0];
+    const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (e >= nval) return;
+    const uint64_t E = entries[e];
+    const int64_t s = (int64_t)(E & kStepMask);
+    const int lane = (int)((E >> kStepBits) & 63u);
+    const unsigned c = (unsigned)((E >> (kStepBits + 6)) & 255u);
+    const uint64_t loc = (E >> (kStepBits + 14)) & 1023u;
     const int ch = (int)(s % G.nch);
     const int64_t row = s / G.nch;
     const int j = (int)(row % G.ry), k = (int)(row / G.ry);
     const int i = ch * 64 + lane;
-    unsigned c = 0, nv = 0;
-    if (i < G.rx) {
-        c = cases[row * G.rx + i];
-        nv = c_nv[c];
-    }
-    const uint32_t incl = wave_incl_scan(nv);
-    if (!nv) return;   // after the wave scan: lanes part here
+    const unsigned nv = c_nv[c];
     const int64_t base = ((int64_t)k * G.ly + j) * G.lx;
     // voxelToMeshKernel: voxelPos = shift * voxelSize + origin
     const P3 vp{(double)i * vx + ox, (double)j * vy + oy, (double)k * vz + oz};
@@ -171,7 +203,7 @@ __device__ inline void emit_step(const float* __restrict__ lat, const McGrid& G,
 #pragma unroll
     for (int q = 0; q < 8; ++q) sv[q] = (double)lat[base + (c_cz[q] * G.ly + c_cy[q]) * G.lx + i + c_cx[q]];
     const uint4 tri = *reinterpret_cast<const uint4*>(&c_tri[c][0]);
-    const uint64_t first = at + incl - nv;
+    const uint64_t first = step_off[s] + loc;
     const double iso = G.iso64;   // MarchingCubes.cu:72-90 works with the caller's binary64 iso
     for (unsigned t = 0; t < nv; t += 3) {
         const uint64_t pos = first + t;
@@ -179,8 +211,8 @@ __device__ inline void emit_step(const float* __restrict__ lat, const McGrid& G,
         for (unsigned q = 0; q < 3; ++q) {
             const unsigned m = t + q;
             const unsigned word = (m >> 2) == 0 ? tri.x : (m >> 2) == 1 ? tri.y : (m >> 2) == 2 ? tri.z : tri.w;
-            const int e = (int)(signed char)(word >> (8 * (m & 3)));
-            const int qa = edge_a(e), qb = edge_b(e);
+            const int ed = (int)(signed char)(word >> (8 * (m & 3)));
+            const int qa = edge_a(ed), qb = edge_b(ed);
             const P3 p = vlerp(corner_pos(vp, vx, vy, vz, qa), corner_pos(vp, vx, vy, vz, qb), pick8(sv, qa),
                                pick8(sv, qb), iso);
             out[3 * (pos + q) + 0] = (float)p.x;
@@ -188,19 +220,6 @@ __device__ inline void emit_step(const float* __restrict__ lat, const McGrid& G,
             out[3 * (pos + q) + 2] = (float)p.z;
         }
     }
-}
-
-__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
-                                                double oz, double vx, double vy, double vz,
-                                                const uint8_t* __restrict__ cases, const uint64_t* __restrict__ step_off,
-                                                const uint32_t* __restrict__ steps,
-                                                const unsigned long long* __restrict__ counts, uint64_t max_verts,
-                                                float* __restrict__ out) {
-    const int64_t nlist = (int64_t)min(counts[0] & kStepMask, (unsigned long long)G.nsteps);
-    const int64_t e = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
-    if (e >= nlist) return;   // whole waves
-    const uint32_t s = steps[e];
-    emit_step(lat, G, ox, oy, oz, vx, vy, vz, cases, step_off[s], (int64_t)s, max_verts, out);
 }
 
 // once per device, also when several host threads make their first call
@@ -304,20 +323,19 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     G.lx = res[0] + 1;
     G.ly = res[1] + 1;
     G.nch = (res[0] + 63) / 64;
-    G.J = 32;
+    G.J = kRows;   // the count pass keeps kRows rows of cases in LDS
     G.njb = (res[1] + G.J - 1) / G.J;
     G.nwaves = (int64_t)G.nch * G.njb * res[2];
     G.nsteps = (int64_t)res[1] * res[2] * G.nch;
     G.iso = (float)iso;
     G.iso64 = iso;
-    s->mc_case.ensure(nvox);
     s->mc_step.ensure(G.nsteps);
     ITS_REQUIRE(G.nsteps < (int64_t)kStepMask, ITS_E_RANGE, "marching-cubes grid has too many rows");
-    s->mc_list.ensure(G.nsteps);
+    s->mc_list.ensure(nvox);   // at most one entry per voxel
     s->mc_valid.ensure(2);
     ITS_HIP(hipMemsetAsync(s->mc_valid.p, 0, sizeof(unsigned long long), st));
     hipLaunchKernelGGL(k_mc_count, dim3((unsigned)((G.nwaves * 64 + kT - 1) / kT)), dim3(kT), 0, st,
-                       (const float*)s->mc_lat.p, G, s->mc_case.p, s->mc_step.p, s->mc_valid.p, s->mc_list.p);
+                       (const float*)s->mc_lat.p, G, s->mc_step.p, s->mc_valid.p, s->mc_list.p);
     ITS_LAUNCH_CHECK();
     // the packed counter and the scan's total come back in one read
     static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "counter width");
@@ -325,17 +343,16 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     unsigned long long back[2];
     ITS_HIP(hipMemcpyAsync(back, s->mc_valid.p, sizeof back, hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
-    const unsigned long long counts = back[0];
+    const int64_t nval = (int64_t)back[0];
     const uint64_t total = back[1];
-    const int64_t nlist = (int64_t)(counts & kStepMask);
     S.num_verts = (int64_t)total;
     const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
     if (h_tri && total) {
         ITS_REQUIRE((int64_t)total <= capacity, ITS_E_RANGE, "triangle buffer too small");
         s->mc_soup.ensure(3 * total);
-        hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((nlist * 64 + kT - 1) / kT)), dim3(kT), 0, st,
+        hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((nval + kT - 1) / kT)), dim3(kT), 0, st,
                            (const float*)s->mc_lat.p, G, origin[0], origin[1], origin[2], vs[0], vs[1], vs[2],
-                           (const uint8_t*)s->mc_case.p, (const uint64_t*)s->mc_step.p, (const uint32_t*)s->mc_list.p,
+                           (const uint64_t*)s->mc_step.p, (const uint64_t*)s->mc_list.p,
                            (const unsigned long long*)s->mc_valid.p, max_verts, s->mc_soup.p);
         ITS_LAUNCH_CHECK();
         ITS_HIP(hipEventRecord(ev[2], st));
@@ -344,7 +361,7 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
         ITS_HIP(hipEventRecord(ev[2], st));
     }
     ITS_HIP(hipStreamSynchronize(st));
-    S.num_valid_voxels = (int64_t)(counts >> kStepBits);
+    S.num_valid_voxels = nval;
     auto t2 = std::chrono::steady_clock::now();
     float ms_lat = 0.f, ms_ext = 0.f;
     ITS_HIP(hipEventElapsedTime(&ms_lat, ev[0], ev[1]));
