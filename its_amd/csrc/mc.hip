@@ -174,19 +174,15 @@ __device__ inline P3 corner_pos(const P3& vp, double vx, double vy, double vz, i
               vp.z + (((kCornerZ >> q) & 1) ? vz : 0.0)};
 }
 
-// pass 2: one thread per voxel with triangles.  Its triangles start at its
-// step's place in the soup (step_off: exclusive scan of the step counts)
-// plus its offset within the step.
-__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
-                                                double oz, double vx, double vy, double vz,
-                                                const uint64_t* __restrict__ step_off,
-                                                const uint64_t* __restrict__ entries,
-                                                const unsigned long long* __restrict__ counts, uint64_t max_verts,
-                                                float* __restrict__ out) {
-    const int64_t nval = (int64_t)counts[0];
-    const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
-    if (e >= nval) return;
-    const uint64_t E = entries[e];
+// pass 2: one thread per voxel with triangles (a fixed grid walks the list,
+// whose length it reads on the device, so the launch needs nothing from the
+// host).  A voxel's triangles start at its step's place in the soup
+// (step_off: exclusive scan of the step counts) plus its offset within the
+// step; no vertex goes at or beyond max_verts, the smaller of the
+// reference's bound and the soup buffer.
+__device__ inline void emit_voxel(const float* __restrict__ lat, const McGrid& G, double ox, double oy, double oz,
+                                  double vx, double vy, double vz, const uint64_t* __restrict__ step_off, uint64_t E,
+                                  uint64_t max_verts, float* __restrict__ out) {
     const int64_t s = (int64_t)(E & kStepMask);
     const int lane = (int)((E >> kStepBits) & 63u);
     const unsigned c = (unsigned)((E >> (kStepBits + 6)) & 255u);
@@ -207,7 +203,7 @@ __global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, M
     const double iso = G.iso64;   // MarchingCubes.cu:72-90 works with the caller's binary64 iso
     for (unsigned t = 0; t < nv; t += 3) {
         const uint64_t pos = first + t;
-        if (pos >= max_verts - 3) break;
+        if (pos + 3 > max_verts) break;
         for (unsigned q = 0; q < 3; ++q) {
             const unsigned m = t + q;
             const unsigned word = (m >> 2) == 0 ? tri.x : (m >> 2) == 1 ? tri.y : (m >> 2) == 2 ? tri.z : tri.w;
@@ -220,6 +216,16 @@ __global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, M
             out[3 * (pos + q) + 2] = (float)p.z;
         }
     }
+}
+__global__ __launch_bounds__(kT) void k_mc_emit(const float* __restrict__ lat, McGrid G, double ox, double oy,
+                                                double oz, double vx, double vy, double vz,
+                                                const uint64_t* __restrict__ step_off,
+                                                const uint64_t* __restrict__ entries,
+                                                const unsigned long long* __restrict__ counts, uint64_t max_verts,
+                                                float* __restrict__ out) {
+    const int64_t nval = (int64_t)counts[0];
+    for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < nval; e += (int64_t)gridDim.x * kT)
+        emit_voxel(lat, G, ox, oy, oz, vx, vy, vz, step_off, entries[e], max_verts, out);
 }
 
 // once per device, also when several host threads make their first call
@@ -337,28 +343,39 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     hipLaunchKernelGGL(k_mc_count, dim3((unsigned)((G.nwaves * 64 + kT - 1) / kT)), dim3(kT), 0, st,
                        (const float*)s->mc_lat.p, G, s->mc_step.p, s->mc_valid.p, s->mc_list.p);
     ITS_LAUNCH_CHECK();
-    // the packed counter and the scan's total come back in one read
+    // the scan's total lands beside the list length
     static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "counter width");
     scan_exclusive_u64_async(s->mc_step.p, s->mc_step.p, G.nsteps, st, (uint64_t*)s->mc_valid.p + 1);
+    const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
+    const unsigned emit_grid =
+        (unsigned)std::min<int64_t>((nvox + kT - 1) / kT, (int64_t)s->ctx->prop.multiProcessorCount * 16);
+    auto emit = [&]() {
+        hipLaunchKernelGGL(k_mc_emit, dim3(emit_grid), dim3(kT), 0, st, (const float*)s->mc_lat.p, G, origin[0],
+                           origin[1], origin[2], vs[0], vs[1], vs[2], (const uint64_t*)s->mc_step.p,
+                           (const uint64_t*)s->mc_list.p, (const unsigned long long*)s->mc_valid.p,
+                           std::min<uint64_t>(max_verts, s->mc_soup.n / 3), s->mc_soup.p);
+        ITS_LAUNCH_CHECK();
+    };
+    // the soup buffer only grows: when it already holds the last soup, the
+    // emit pass follows the scan with no host round trip, and it is run again
+    // only when this soup needs more room
+    const bool want = h_tri != nullptr;
+    if (want && s->mc_soup.n >= 3) emit();
+    ITS_HIP(hipEventRecord(ev[2], st));
     unsigned long long back[2];
     ITS_HIP(hipMemcpyAsync(back, s->mc_valid.p, sizeof back, hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
     const int64_t nval = (int64_t)back[0];
     const uint64_t total = back[1];
     S.num_verts = (int64_t)total;
-    const uint64_t max_verts = 18ull * (uint64_t)nvox;   // maxVerts = nVoxels * 18
-    if (h_tri && total) {
+    if (want && total) {
         ITS_REQUIRE((int64_t)total <= capacity, ITS_E_RANGE, "triangle buffer too small");
-        s->mc_soup.ensure(3 * total);
-        hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((nval + kT - 1) / kT)), dim3(kT), 0, st,
-                           (const float*)s->mc_lat.p, G, origin[0], origin[1], origin[2], vs[0], vs[1], vs[2],
-                           (const uint64_t*)s->mc_step.p, (const uint64_t*)s->mc_list.p,
-                           (const unsigned long long*)s->mc_valid.p, max_verts, s->mc_soup.p);
-        ITS_LAUNCH_CHECK();
-        ITS_HIP(hipEventRecord(ev[2], st));
+        if (3 * total > s->mc_soup.n) {
+            s->mc_soup.ensure(3 * total);
+            emit();
+            ITS_HIP(hipEventRecord(ev[2], st));
+        }
         ITS_HIP(hipMemcpyAsync(h_tri, s->mc_soup.p, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
-    } else {
-        ITS_HIP(hipEventRecord(ev[2], st));
     }
     ITS_HIP(hipStreamSynchronize(st));
     S.num_valid_voxels = nval;

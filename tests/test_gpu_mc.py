@@ -200,3 +200,23 @@ def test_octree_from_vertices_errors(ctx):
     off[-1] += 0.3 * widths[owner[-1]]          # off its level's lattice
     with pytest.raises(_capi.ItsError):
         _capi.Octree.from_vertices(ctx, off, owner, widths)
+
+
+def test_mc_soup_buffer_reuse(ctx):
+    """The octree keeps its soup buffer between extractions and emits into it
+    before the soup's size is known; a soup that does not fit is emitted
+    again into a larger buffer.  Grids that alternate between small and large
+    soups give, call after call, the soup of a fresh octree."""
+    F = fitted("icosphere", 64)
+    c = F.case
+    fresh = {}
+    for res in (16, 64, 40):
+        g = _gpu(ctx, F)
+        fresh[res] = g.marching_cubes(c.origin, c.width, (res, res, res), 0.0)[0]
+        g.close()
+    g = _gpu(ctx, F)
+    for res in (16, 64, 16, 40, 64, 64, 16):
+        tri, st = g.marching_cubes(c.origin, c.width, (res, res, res), 0.0)
+        assert st["num_verts"] == len(fresh[res])
+        assert np.array_equal(tri, fresh[res]), res
+    g.close()
