@@ -42,18 +42,15 @@ __constant__ unsigned char c_cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 // The voxel grid in the reference's voxel order (x fastest, then y, then z),
 // cut into steps of 64 voxels along x: step s = (j + ry * k) * nch + chunk.
-// The count pass gives each wave one 64-voxel column chunk over J rows of
-// one z-slab: walking +y, the lattice rows at j+1 become those at j, so a
-// step loads two new rows (x and x+1 for the planes k and k+1), prefetched
-// one row ahead.  It writes each step's vertex count; a scan of the step
-// counts gives each step its place in the soup.
+// The count pass writes each step's vertex count; a scan of the step counts
+// gives each step its place in the soup.
 struct McGrid {
     int rx, ry, rz;
     int64_t lx, ly;   // lattice points per axis in x, y (res + 1)
     int nch;          // 64-voxel chunks per row
-    int J;            // rows per count wave
-    int njb;          // ceil(ry / J)
-    int64_t nwaves;   // count waves: nch * njb * rz
+    int njb;          // ceil(ry / kJ): row blocks of the count waves
+    int nkb;          // ceil(rz / kK): slab blocks of the count waves
+    int64_t nwaves;   // count waves: nch * njb * nkb
     int64_t nsteps;   // ry * rz * nch
     float iso;      // the cube cases compare the binary32 lattice with this
     double iso64;   // the caller's iso value: vertexLerp's snaps and t
@@ -66,63 +63,97 @@ __device__ inline unsigned case8(const float (&v)[8], float iso) {
     return c;
 }
 
-// pass 1: vertices per step and the list of voxels with triangles.  Per row
-// step a wave forms the 64 cube cases; a voxel has triangles exactly when its
-// case is neither 0 nor 255 (every other case of the table has some), so
-// only rows with such a voxel look up vertex counts and sum them.  The cases
-// wait in LDS while the wave walks its rows; the wave then reserves one list
-// slot per voxel with triangles (one atomic) and, for the rows that have
-// them, forms each voxel's offset within its step (a wave prefix sum) and
-// writes its entry: step (kStepBits bits), lane (6), case (8), offset within
-// the step (10).  The list order depends on timing; each voxel's place in
-// the soup is its step's place (the scan of step_verts) plus its offset, so
-// the output does not.
-constexpr int kRows = 32;   // rows per count wave
+// pass 1: vertices per step and the list of voxels with triangles.  A wave
+// takes one 64-voxel chunk of x over kJ rows (y) and kK slabs (z) and walks
+// +z: each lattice plane of its kJ + 1 rows is loaded once (the x + 1
+// neighbour comes from the next lane; the last lane loads its own), and
+// plane z + 1 becomes plane z of the next slab.  A voxel has triangles
+// exactly when its case is neither 0 nor 255 (every other case of the table
+// has some), so only rows with such a voxel look up vertex counts and sum
+// them.  The cases wait in LDS while the wave walks; the wave then reserves
+// one list slot per voxel with triangles (one atomic) and, for the rows that
+// have them, forms each voxel's offset within its step (a wave prefix sum)
+// and writes its entry: step (kStepBits bits), lane (6), case (8), offset
+// within the step (10).  The list order depends on timing; each voxel's
+// place in the soup is its step's place (the scan of step_verts) plus its
+// offset, so the output does not.
+constexpr int kJ = 4;              // rows (y) per count wave
+constexpr int kK = 8;              // slabs (z) per count wave
+constexpr int kRows = kJ * kK;     // step rows per count wave (at most 32: a bit mask)
 constexpr int kWaves = kT / 64;
 __device__ inline uint64_t pack_entry(int64_t s, int lane, unsigned c, unsigned off) {
     return (uint64_t)s | ((uint64_t)lane << kStepBits) | ((uint64_t)c << (kStepBits + 6)) |
            ((uint64_t)off << (kStepBits + 14));
 }
 __device__ inline bool has_triangles(unsigned c) { return c != 0u && c != 255u; }
+// lattice rows y = j0 .. j0 + kJ (clamped to the lattice) of plane z at x
+// (P0) and x + 1 (P1)
+// (all loads are issued before the first shuffle waits for one)
+__device__ inline void load_plane(const float* __restrict__ lat, const McGrid& G, int z, int j0, int xl, int xlast,
+                                  int lane, float (&P0)[kJ + 1], float (&P1)[kJ + 1]) {
+    float E[kJ + 1];
+#pragma unroll
+    for (int r = 0; r <= kJ; ++r) {
+        const int64_t base = ((int64_t)z * G.ly + min(j0 + r, G.ry)) * G.lx;
+        P0[r] = lat[base + xl];
+        E[r] = lat[base + xlast];   // the same point for every lane
+    }
+#pragma unroll
+    for (int r = 0; r <= kJ; ++r) {
+        const float nb = __shfl_down(P0[r], 1, 64);
+        P1[r] = lane == 63 ? E[r] : nb;
+    }
+}
 __global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, McGrid G,
                                                  uint64_t* __restrict__ step_verts, unsigned long long* counts,
                                                  uint64_t* __restrict__ entries) {
     __shared__ uint8_t s_case[kWaves][kRows][64];
+    __shared__ uint8_t s_nv[256];   // vertex counts per case, read per lane
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t w = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+    static_assert(kT == 256, "one table entry per thread");
+    s_nv[threadIdx.x] = c_nv[threadIdx.x];
+    __syncthreads();
     if (w >= G.nwaves) return;   // whole waves only
     const int ch = (int)(w % G.nch);
     const int64_t rest = w / G.nch;
-    const int jb = (int)(rest % G.njb), k = (int)(rest / G.njb);
-    const int j0 = jb * G.J, j1 = min(j0 + G.J, G.ry);
+    const int jb = (int)(rest % G.njb), kb = (int)(rest / G.njb);
+    const int j0 = jb * kJ, j1 = min(j0 + kJ, G.ry);
+    const int k0 = kb * kK, k1 = min(k0 + kK, G.rz);
     const int i = ch * 64 + lane;
     const bool in = i < G.rx;
-    const int ii = in ? i : G.rx - 1;   // lanes past the row read in bounds and count nothing
-    const int64_t p0 = (int64_t)k * G.ly * G.lx + ii, p1 = p0 + G.ly * G.lx;
-    // lattice rows at j (a: plane k, b: plane k + 1), each at x and x + 1
-    float a0 = lat[p0 + j0 * G.lx], a1 = lat[p0 + j0 * G.lx + 1];
-    float b0 = lat[p1 + j0 * G.lx], b1 = lat[p1 + j0 * G.lx + 1];
-    float na0 = lat[p0 + (j0 + 1) * G.lx], na1 = lat[p0 + (j0 + 1) * G.lx + 1];
-    float nb0 = lat[p1 + (j0 + 1) * G.lx], nb1 = lat[p1 + (j0 + 1) * G.lx + 1];
+    // lattice x of this lane (lanes past the row read in bounds and count
+    // nothing), and x + 1 of the last lane
+    const int xl = min(i, G.rx), xlast = min(ch * 64 + 64, G.rx);
+    float C0[kJ + 1], C1[kJ + 1], N0[kJ + 1], N1[kJ + 1];
+    load_plane(lat, G, k0, j0, xl, xlast, lane, C0, C1);
     unsigned nval = 0;
-    uint32_t rows = 0;   // bit j - j0: row j has a voxel with triangles
-    for (int j = j0; j < j1; ++j) {
-        const int64_t jn = min(j + 2, G.ry) * G.lx;
-        const float f0 = lat[p0 + jn], f1 = lat[p0 + jn + 1], g0 = lat[p1 + jn], g1 = lat[p1 + jn + 1];
-        // the reference's corner order: (x,y,z) = (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1)
-        const float v[8] = {a0, na0, na1, a1, b0, nb0, nb1, b1};
-        const unsigned c = in ? case8(v, G.iso) : 0u;
-        s_case[wv][j - j0][lane] = (uint8_t)c;
-        const uint64_t bal = __ballot(has_triangles(c));
-        unsigned sum = 0;
-        if (bal) {
-            sum = wave_sum(has_triangles(c) ? (unsigned)c_nv[c] : 0u);
-            nval += __popcll(bal);
-            rows |= 1u << (j - j0);
+    uint32_t rows = 0;   // bit (k - k0) kJ + (j - j0): that step row has a voxel with triangles
+    for (int k = k0; k < k1; ++k) {
+        load_plane(lat, G, k + 1, j0, xl, xlast, lane, N0, N1);
+#pragma unroll
+        for (int r = 0; r < kJ; ++r) {
+            const int j = j0 + r;
+            if (j >= j1) break;   // uniform
+            // the reference's corner order: (x,y,z) = (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1)
+            const float v[8] = {C0[r], C0[r + 1], C1[r + 1], C1[r], N0[r], N0[r + 1], N1[r + 1], N1[r]};
+            const unsigned c = in ? case8(v, G.iso) : 0u;
+            const int row = (k - k0) * kJ + r;
+            s_case[wv][row][lane] = (uint8_t)c;
+            const uint64_t bal = __ballot(has_triangles(c));
+            unsigned sum = 0;
+            if (bal) {
+                sum = wave_sum(has_triangles(c) ? (unsigned)s_nv[c] : 0u);
+                nval += __popcll(bal);
+                rows |= 1u << row;
+            }
+            if (lane == 0) step_verts[((int64_t)k * G.ry + j) * G.nch + ch] = sum;
         }
-        if (lane == 0) step_verts[((int64_t)k * G.ry + j) * G.nch + ch] = sum;
-        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
-        na0 = f0; na1 = f1; nb0 = g0; nb1 = g1;
+#pragma unroll
+        for (int r = 0; r <= kJ; ++r) {
+            C0[r] = N0[r];
+            C1[r] = N1[r];
+        }
     }
     if (!nval) return;
     unsigned long long slot = 0;
@@ -131,14 +162,15 @@ __global__ __launch_bounds__(kT) void k_mc_count(const float* __restrict__ lat, 
     const uint64_t below = (1ull << lane) - 1ull;
     // every lane reads back only the cases it wrote itself
     while (rows) {
-        const int r = __builtin_ctz(rows);
+        const int row = __builtin_ctz(rows);
         rows &= rows - 1u;
-        const unsigned c = s_case[wv][r][lane];
+        const unsigned c = s_case[wv][row][lane];
         const bool act = has_triangles(c);
-        const unsigned nv = act ? (unsigned)c_nv[c] : 0u;
+        const unsigned nv = act ? (unsigned)s_nv[c] : 0u;
         const unsigned incl = wave_incl_scan(nv);
         const uint64_t bal = __ballot(act);
-        if (act) entries[slot + __popcll(bal & below)] = pack_entry(((int64_t)k * G.ry + j0 + r) * G.nch + ch, lane, c,
+        const int k = k0 + row / kJ, j = j0 + row % kJ;
+        if (act) entries[slot + __popcll(bal & below)] = pack_entry(((int64_t)k * G.ry + j) * G.nch + ch, lane, c,
                                                                     incl - nv);
         slot += __popcll(bal);
     }
@@ -329,9 +361,9 @@ void mc_extract(its_svo s, const double origin[3], const double width[3], const 
     G.lx = res[0] + 1;
     G.ly = res[1] + 1;
     G.nch = (res[0] + 63) / 64;
-    G.J = kRows;   // the count pass keeps kRows rows of cases in LDS
-    G.njb = (res[1] + G.J - 1) / G.J;
-    G.nwaves = (int64_t)G.nch * G.njb * res[2];
+    G.njb = (res[1] + kJ - 1) / kJ;
+    G.nkb = (res[2] + kK - 1) / kK;
+    G.nwaves = (int64_t)G.nch * G.njb * G.nkb;
     G.nsteps = (int64_t)res[1] * res[2] * G.nch;
     G.iso = (float)iso;
     G.iso64 = iso;
