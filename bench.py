@@ -184,6 +184,12 @@ def main():
                  "bytes_assembled_per_iteration": ls["bytes_assembled_per_iteration"],
                  "assembled_equivalent": ls["bytes_assembled_per_iteration"] / t_it / 1e9,
                  "frac_assembled_equivalent": ls["bytes_assembled_per_iteration"] / t_it / 1e9 / HBM_PEAK_GBS}
+    # measured HBM traffic of one iteration (rocprofv3 counters, summed over
+    # the iteration's kernels)
+    lpmc = os.path.join(ROOT, "profiles", "lscg_knot256_traffic.json")
+    if os.path.exists(lpmc):
+        with open(lpmc) as fh:
+            lscg_roof["traffic"] = json.load(fh).get("bytes_per_iteration")
     log(f"[bench] lscg roofline: {t_it * 1e6:.1f} us per iteration, matrix-free "
         f"{lscg_roof['achieved']:.0f} GB/s, assembled-equivalent {lscg_roof['assembled_equivalent']:.0f} GB/s")
 
