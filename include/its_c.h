@@ -343,6 +343,10 @@ int its_batch_eval_device_noalloc(its_batch b, const float* d_x, const float* d_
  *   walks its own query (default 1).
  * ITS_TUNE_BATCH_MAX_VERTS: meshes of a batch with at most this many
  *   unknowns share the one-launch fit (default 65536).
+ * ITS_TUNE_SDF_RAY: 1 = for a closed, consistently oriented mesh the sign of
+ *   the signed distance comes from the crossings of a ray where exact
+ *   orientation tests decide it (the winding-number fan sum for the other
+ *   points); 0 = the fan sum for every point (default 1; same results).
  * No reference counterpart (the reference hard-codes its launch shapes). */
 #define ITS_TUNE_DENSE_CELLS 0
 #define ITS_TUNE_EVAL_BLOCKS_PER_CU 1
@@ -350,7 +354,8 @@ int its_batch_eval_device_noalloc(its_batch b, const float* d_x, const float* d_
 #define ITS_TUNE_EVAL_ILP 3
 #define ITS_TUNE_EVAL_QUEUE 4
 #define ITS_TUNE_BATCH_MAX_VERTS 5
-#define ITS_TUNE_COUNT 6
+#define ITS_TUNE_SDF_RAY 6
+#define ITS_TUNE_COUNT 7
 int its_tune_set(int key, int64_t value);
 int its_tune_get(int key, int64_t* value);
 

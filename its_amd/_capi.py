@@ -20,6 +20,7 @@ ITS_MAX_LEVELS = 12
 EVAL_SUPPORT, EVAL_DENSE, EVAL_RAW = 0, 1, 2
 TUNE_DENSE_CELLS, TUNE_EVAL_BLOCKS_PER_CU, TUNE_COARSE_VERTS, TUNE_EVAL_ILP, TUNE_EVAL_QUEUE = 0, 1, 2, 3, 4
 TUNE_BATCH_MAX_VERTS = 5
+TUNE_SDF_RAY = 6
 NOALLOC_MAX_MESHES, NOALLOC_MAX_ITEMS, NOALLOC_CHUNK = 1024, 65536, 16384
 E_INVALID, E_HIP, E_NOMEM, E_STATE, E_RANGE = -1, -2, -3, -4, -5
 

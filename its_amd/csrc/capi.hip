@@ -16,7 +16,7 @@ namespace its {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
-static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 64, 2048, 1, 1, 65536};
+static int64_t g_tune[ITS_TUNE_COUNT] = {32768, 64, 2048, 1, 1, 65536, 1};
 int64_t tune_get(int key) { return g_tune[key]; }
 }  // namespace its
 
@@ -1019,8 +1019,9 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, dou
 int its_tune_set(int key, int64_t value) {
     ITS_API_BEGIN
     ITS_REQUIRE(key >= 0 && key < ITS_TUNE_COUNT, ITS_E_INVALID, "unknown tunable");
-    ITS_REQUIRE(value >= (key == ITS_TUNE_EVAL_QUEUE ? 0 : 1), ITS_E_RANGE, "tunable out of range");
-    if (key == ITS_TUNE_EVAL_QUEUE) ITS_REQUIRE(value <= 1, ITS_E_RANGE, "queue switch is 0 or 1");
+    const bool is_switch = key == ITS_TUNE_EVAL_QUEUE || key == ITS_TUNE_SDF_RAY;
+    ITS_REQUIRE(value >= (is_switch ? 0 : 1), ITS_E_RANGE, "tunable out of range");
+    if (is_switch) ITS_REQUIRE(value <= 1, ITS_E_RANGE, "a switch is 0 or 1");
     if (key == ITS_TUNE_COARSE_VERTS) ITS_REQUIRE(value <= 12288, ITS_E_RANGE, "coarse vertex bound above 12288");
     if (key == ITS_TUNE_EVAL_BLOCKS_PER_CU) ITS_REQUIRE(value <= 64, ITS_E_RANGE, "at most 64 workgroups per CU");
     if (key == ITS_TUNE_EVAL_ILP) ITS_REQUIRE(value == 1 || value == 2 || value == 4, ITS_E_RANGE, "ILP must be 1, 2 or 4");

@@ -32,6 +32,20 @@
 //     which is exact for open, non-manifold or inconsistently oriented
 //     meshes too.  A node's fan is stored only when it has at most as many
 //     edges as the node has triangles; otherwise the walk descends.
+//   - Winding number of a closed mesh, by a ray: when every undirected edge
+//     has exactly two half edges of opposite directions, the mesh is an
+//     oriented closed surface, and its winding number at p is the signed
+//     number of crossings of any ray from p (self-intersecting and nested
+//     surfaces included).  The +z ray is tested against the triangles whose
+//     boxes it meets: p is inside the xy projection of a triangle when its
+//     three edge orientations agree, and the crossing lies above p when the
+//     orientation of the tetrahedron (a, b, c, p) has that same sign.  Each
+//     orientation is a binary64 determinant whose sign is taken only when it
+//     exceeds the rounding bound of its terms (the static filters of
+//     Shewchuk, Adaptive Precision Floating-Point Arithmetic and Fast Robust
+//     Geometric Predicates, 1997); a point with any undecided orientation
+//     (on or within rounding of an edge line or a triangle plane) takes the
+//     fan walk above.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
@@ -238,6 +252,7 @@ struct TreeView {
     const float* fan;        // 6 per fan edge: a - centre, b - centre (centre of its node's box)
     const uint32_t* cell_lb; // per cell_bits-bit code prefix C: first sorted triangle with code >= C's range
     int cell_bits;
+    int closed;              // every undirected edge: exactly two half edges, opposite directions
 };
 
 // -------------------------------------------------------------- build -----
@@ -378,6 +393,20 @@ __device__ inline bool edge_head(const uint64_t* keys, int64_t i, uint64_t none)
     return keys[i] != none && (i == 0 || keys[i] != keys[i - 1]);
 }
 
+// *open = 1 unless every undirected edge has exactly two half edges of
+// opposite directions (a degenerate edge also opens the mesh)
+__global__ void k_closed_check(const uint64_t* keys, const uint32_t* vals, int64_t M, uint64_t none, int* open) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    if (keys[i] == none) {
+        *open = 1;
+        return;
+    }
+    if (!edge_head(keys, i, none)) return;
+    const bool pair = i + 1 < M && keys[i + 1] == keys[i] && (i + 2 >= M || keys[i + 2] != keys[i]);
+    if (!pair || ((vals[i] ^ vals[i + 1]) & 1u) == 0u) *open = 1;
+}
+
 __global__ void k_fan_node_count(const uint64_t* keys, const uint32_t* vals, int64_t M, Shape s, uint64_t none,
                                  uint32_t* node_cnt) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -515,8 +544,9 @@ struct TriTree {
     DBuf<int32_t> tid;
     DBuf<uint32_t> code, cell_lb;
     int cell_bits = 0;
+    bool closed = false;
     TreeView view() const {
-        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p, cell_lb.p, cell_bits};
+        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p, cell_lb.p, cell_bits, closed ? 1 : 0};
     }
 };
 
@@ -607,6 +637,11 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     ITS_LAUNCH_CHECK();
     radix_sort_pairs(hkeys, hvals, M, std::min(64, 2 * vb), st);
 
+    DBuf<int> is_open(1);
+    ITS_HIP(hipMemsetAsync(is_open.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_closed_check, dim3(blocks_for(M, kT)), dim3(kT), 0, st, (const uint64_t*)hkeys.p,
+                       (const uint32_t*)hvals.p, M, none, is_open.p);
+    ITS_LAUNCH_CHECK();
     DBuf<uint32_t> node_cnt(tr.nodes);
     ITS_HIP(hipMemsetAsync(node_cnt.p, 0, tr.nodes * sizeof(uint32_t), st));
     hipLaunchKernelGGL(k_fan_node_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
@@ -615,8 +650,11 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     DBuf<uint8_t> keep(tr.nodes);
     {
         std::vector<uint32_t> h_cnt(tr.nodes);
+        int h_open = 1;
         ITS_HIP(hipMemcpyAsync(h_cnt.data(), node_cnt.p, tr.nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        ITS_HIP(hipMemcpyAsync(&h_open, is_open.p, sizeof(int), hipMemcpyDeviceToHost, st));
         ITS_HIP(hipStreamSynchronize(st));
+        tr.closed = h_open == 0;
         const std::vector<uint8_t> h_keep = fan_choice(s, std::min(kLeafLevel, (int)s.H), h_cnt);
         ITS_HIP(hipMemcpyAsync(keep.p, h_keep.data(), tr.nodes, hipMemcpyHostToDevice, st));
         ITS_HIP(hipStreamSynchronize(st));
@@ -929,7 +967,164 @@ __device__ float wind_dfs(const TreeView& t, int leaf, int lane, const double p[
     return acc;
 }
 
-__global__ __launch_bounds__(kT) void k_sdf_wind(TreeView t, const double* pts, int64_t N, double* out) {
+// Orientation signs with static error filters (Shewchuk 1997, the first
+// stage of orient2d and orient3d): +1 or -1 when the binary64 determinant
+// exceeds the bound on its rounding error, 0 when the sign is not certain.
+// Nothing here may be contracted into fused multiply-adds: the bounds count
+// every rounding of the products and sums as written.
+constexpr double kO2dBound = 3.3306690738754716e-16;   // (3 + 16 eps) eps, eps = 2^-53
+constexpr double kO3dBound = 7.7715611723761027e-16;   // (7 + 56 eps) eps
+// (a - p) x (b - p), z component: > 0 when p lies left of a -> b in xy
+__device__ inline int o2d_sign(double ax, double ay, double bx, double by, double px, double py) {
+#pragma clang fp contract(off)
+    const double l = (ax - px) * (by - py);
+    const double r = (ay - py) * (bx - px);
+    const double det = l - r;
+    const double bound = kO2dBound * (fabs(l) + fabs(r));
+    return det > bound ? 1 : (-det > bound ? -1 : 0);
+}
+// (a - p) . ((b - p) x (c - p))
+__device__ inline int o3d_sign(const double* a, const double* b, const double* c, const double p[3]) {
+#pragma clang fp contract(off)
+    const double adx = a[0] - p[0], ady = a[1] - p[1], adz = a[2] - p[2];
+    const double bdx = b[0] - p[0], bdy = b[1] - p[1], bdz = b[2] - p[2];
+    const double cdx = c[0] - p[0], cdy = c[1] - p[1], cdz = c[2] - p[2];
+    const double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const double cdxady = cdx * ady, adxcdy = adx * cdy;
+    const double adxbdy = adx * bdy, bdxady = bdx * ady;
+    const double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    const double perm = (fabs(bdxcdy) + fabs(cdxbdy)) * fabs(adz) + (fabs(cdxady) + fabs(adxcdy)) * fabs(bdz) +
+                        (fabs(adxbdy) + fabs(bdxady)) * fabs(cdz);
+    const double bound = kO3dBound * perm;
+    return det > bound ? 1 : (-det > bound ? -1 : 0);
+}
+
+// Signed crossing of the +z ray from p with one triangle: +1 / -1 (the sign
+// of its xy orientation) when the ray crosses it above p, else 0; unsure is
+// set when an orientation that decides this is not certain.
+__device__ inline int tri_ray(const double* T, const double p[3], bool& unsure) {
+    const double* a = T;
+    const double* b = T + 3;
+    const double* c = T + 6;
+    // outside the xy bounding rectangle, or wholly below p: no crossing
+    if (p[0] < fmin(a[0], fmin(b[0], c[0])) || p[0] > fmax(a[0], fmax(b[0], c[0])) ||
+        p[1] < fmin(a[1], fmin(b[1], c[1])) || p[1] > fmax(a[1], fmax(b[1], c[1])) ||
+        fmax(a[2], fmax(b[2], c[2])) < p[2])
+        return 0;
+    const int s1 = o2d_sign(a[0], a[1], b[0], b[1], p[0], p[1]);
+    const int s2 = o2d_sign(b[0], b[1], c[0], c[1], p[0], p[1]);
+    const int s3 = o2d_sign(c[0], c[1], a[0], a[1], p[0], p[1]);
+    const bool pos = s1 > 0 || s2 > 0 || s3 > 0, neg = s1 < 0 || s2 < 0 || s3 < 0;
+    if (pos && neg) return 0;   // two certain signs disagree: outside the projection
+    if (s1 == 0 || s2 == 0 || s3 == 0) {
+        unsure = true;
+        return 0;
+    }
+    // inside the projection; s1 is the xy orientation of (a, b, c).  The
+    // ray meets the plane at t = ((a - p) . n) / n_z, n = (b - a) x (c - a),
+    // and (a - p) . n is the orientation of (a, b, c, p): above p when it
+    // has the sign of n_z
+    if (fmin(a[2], fmin(b[2], c[2])) > p[2]) return s1;
+    const int s4 = o3d_sign(a, b, c, p);
+    if (s4 == 0) {
+        unsure = true;
+        return 0;
+    }
+    return s4 == s1 ? s1 : 0;
+}
+
+// Winding numbers of a closed mesh by the +z ray (see the file comment):
+// the wavefront walks the tree level by level, keeping the nodes whose
+// boxes meet the ray, then tests the triangles of the leaves it reaches,
+// one per lane.  decided[i] = 1 and the sign is applied when every
+// orientation was certain and the lists fitted; otherwise decided[i] = 0
+// and the fan walk takes the point.
+constexpr int kRayFront = 128, kRayLeaves = 64;
+__device__ inline bool box_on_ray(const double p[3], float4 lo, float4 hi) {
+    return p[0] >= (double)lo.x && p[0] <= (double)hi.x && p[1] >= (double)lo.y && p[1] <= (double)hi.y &&
+           p[2] <= (double)hi.z;
+}
+__global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, int64_t N, double* out,
+                                               uint8_t* decided) {
+    __shared__ uint32_t fk[2][kWavesPerBlock][kRayFront];
+    __shared__ uint32_t leaves[kWavesPerBlock][kRayLeaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+    const int leaf = std::min(kLeafLevel, (int)t.s.H);
+    for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
+        const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        bool unsure = !(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]));
+        int n = 0, cur = 0, nl = 0;
+        if (!unsure) {
+            const NodeRec root = load_node(t, t.s.off[t.s.H]);
+            n = box_on_ray(p, root.lo, root.hi) ? 1 : 0;
+        }
+        lds_fence();
+        if (lane == 0) fk[0][w][0] = 0u;
+        lds_fence();
+        for (int h = t.s.H; n > 0; --h) {
+            if (h <= leaf) {   // every node of this level is a leaf
+                if (nl + n > kRayLeaves) {
+                    unsure = true;
+                    break;
+                }
+                for (int j = lane; j < n; j += 64) leaves[w][nl + j] = fk[cur][w][j];
+                nl += n;
+                lds_fence();
+                break;
+            }
+            const int64_t coff = t.s.off[h - 1];
+            const int cnext = t.s.count[h - 1];
+            int m = 0;
+            for (int j0 = 0; j0 < n; j0 += 64) {
+                const int j = j0 + lane;
+                bool keep0 = false, keep1 = false;
+                uint32_t c0 = 0;
+                if (j < n) {
+                    c0 = 2u * fk[cur][w][j];
+                    const NodeRec r0 = load_node(t, (int)(coff + c0));
+                    keep0 = box_on_ray(p, r0.lo, r0.hi);
+                    if ((int)c0 + 1 < cnext) {
+                        const NodeRec r1 = load_node(t, (int)(coff + c0 + 1));
+                        keep1 = box_on_ray(p, r1.lo, r1.hi);
+                    }
+                }
+                const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+                const int tot = __popcll(m0) + __popcll(m1);
+                if (m + tot > kRayFront) {
+                    unsure = true;
+                    break;
+                }
+                const int pos = m + __popcll(m0 & below) + __popcll(m1 & below);
+                if (keep0) fk[cur ^ 1][w][pos] = c0;
+                if (keep1) fk[cur ^ 1][w][pos + (keep0 ? 1 : 0)] = c0 + 1u;
+                m += tot;
+            }
+            lds_fence();
+            if (unsure) break;
+            cur ^= 1;
+            n = m;
+        }
+        int cross = 0;
+        bool lane_unsure = false;
+        if (!unsure)
+            for (int l = 0; l < nl; ++l) {
+                const int64_t k = leaves[w][l];
+                const int64_t j = node_first(leaf, k) + lane;
+                if (j < node_last(t.s, leaf, k)) cross += tri_ray(t.tris + 9 * j, p, lane_unsure);
+            }
+        unsure = unsure || __ballot(lane_unsure) != 0ull;
+        const int wn = wave_sum(cross);
+        if (lane == 0) {
+            decided[i] = unsure ? 0 : 1;
+            if (!unsure && wn >= 1) out[i] = -out[i];   // winding number > 1/2
+        }
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_sdf_wind(TreeView t, const double* pts, int64_t N, double* out,
+                                                 const uint8_t* decided) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kWindFront];   // nodes of the level being classified
     __shared__ FanItem fans[kWavesPerBlock][kWindFans];
     __shared__ uint32_t leaves[kWavesPerBlock][kWindLeaves];
@@ -938,6 +1133,7 @@ __global__ __launch_bounds__(kT) void k_sdf_wind(TreeView t, const double* pts, 
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
     for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
+        if (decided && decided[i]) continue;   // the ray settled its sign
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
         int n = 1, cur = 0, nf = 0, nl = 0;
         bool overflow = false;
@@ -1021,7 +1217,15 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
     const int64_t wblocks = std::min<int64_t>((N + kWavesPerBlock - 1) / kWavesPerBlock, 8 * (int64_t)cus);
     hipLaunchKernelGGL(k_sdf_dist, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, d_tri);
     ITS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sdf_wind, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out);
+    // closed mesh: signs by the ray where it decides them, the fan walk for the rest
+    DBuf<uint8_t> decided;
+    if (tr.closed && tune_get(ITS_TUNE_SDF_RAY)) {
+        decided.alloc(N);
+        hipLaunchKernelGGL(k_sdf_ray, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, decided.p);
+        ITS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_sdf_wind, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
+                       (const uint8_t*)decided.p);
     ITS_LAUNCH_CHECK();
     ITS_HIP(hipStreamSynchronize(st));
 }

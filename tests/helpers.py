@@ -130,3 +130,44 @@ def bbox_known_answers():
         out.append((V, o - eps, e + eps, (e + eps) - (o - eps)))
     return out
 
+
+
+def exact_winding_number(V, F, p, axis=0):
+    """Winding number of a closed, consistently oriented triangle mesh at p,
+    exactly: the signed count of crossings of the ray from p along +axis,
+    every orientation evaluated in rational arithmetic (fractions) from the
+    binary64 coordinates.  None when the ray meets an edge or p lies on a
+    triangle's plane where it crosses (then another axis decides)."""
+    from fractions import Fraction as Q
+
+    V = np.asarray(V, np.float64)
+    u, v = (axis + 1) % 3, (axis + 2) % 3   # (u, v, axis) is a right-handed frame
+    P = [Q(float(x)) for x in p]
+    T = V[np.asarray(F)]
+    # triangles whose (u, v) box holds p and that reach beyond p along axis
+    lo, hi = T.min(1), T.max(1)
+    cand = np.nonzero((lo[:, u] <= p[u]) & (hi[:, u] >= p[u]) & (lo[:, v] <= p[v]) & (hi[:, v] >= p[v]) &
+                      (hi[:, axis] >= p[axis]))[0]
+
+    def o2(a, b):
+        return (a[u] - P[u]) * (b[v] - P[v]) - (a[v] - P[v]) * (b[u] - P[u])
+
+    w = 0
+    for t in cand:
+        a, b, c = ([Q(float(x)) for x in T[t][k]] for k in range(3))
+        s = [o2(a, b), o2(b, c), o2(c, a)]
+        if any(x > 0 for x in s) and any(x < 0 for x in s):
+            continue
+        if any(x == 0 for x in s):
+            return None
+        sg = 1 if s[0] > 0 else -1
+        ad = [a[k] - P[k] for k in range(3)]
+        bd = [b[k] - P[k] for k in range(3)]
+        cd = [c[k] - P[k] for k in range(3)]
+        n = [bd[(k + 1) % 3] * cd[(k + 2) % 3] - bd[(k + 2) % 3] * cd[(k + 1) % 3] for k in range(3)]
+        det = sum(ad[k] * n[k] for k in range(3))
+        if det == 0:
+            return None
+        if (det > 0) == (sg > 0):
+            w += sg
+    return w
