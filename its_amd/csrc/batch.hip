@@ -12,14 +12,12 @@
 // Every scalar of the recurrence is computed by all threads from the same
 // block sums, so all threads take the same branches and reach every barrier.
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cstring>
-#include <exception>
-#include <thread>
 #include <vector>
 
 #include "fit.hpp"
+#include "lscg_batch.hpp"
 
 namespace its {
 namespace {
@@ -248,20 +246,15 @@ struct its_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t side = nullptr;       // the one-launch fit runs here, beside the large meshes
-    static constexpr int kWorkers = 3;   // large meshes: one host thread and stream each
-    hipStream_t work[kWorkers] = {};
     std::vector<its::FitDesc> host;   // the descriptors the graph was captured with
+    its::ManySolve large;             // the large meshes' solve, on the context stream
     ~its_batch_s() {
         // no stream may still use the buffers freed below
         if (side) (void)hipStreamSynchronize(side);
-        for (hipStream_t w : work)
-            if (w) (void)hipStreamSynchronize(w);
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (side) (void)hipStreamDestroy(side);
-        for (hipStream_t w : work)
-            if (w) (void)hipStreamDestroy(w);
     }
 };
 
@@ -282,12 +275,6 @@ void batch_add(its_batch b, its_svo s) {
     b->svos.push_back(s);
 }
 
-// Fit every mesh of the batch.  Meshes with at most ITS_TUNE_BATCH_MAX_VERTS
-// unknowns are solved together by the one-launch kernel (graph-captured,
-// replayed on later calls) on the batch's own stream; larger meshes, whose
-// solve on one workgroup would bound the launch, run at the same time on the
-// context stream through the multi-workgroup solver.  seconds: wall time of
-// the whole fit.
 // Octrees built on other contexts of the batch's device: their streams
 // finish before the batch's streams read the octrees.
 static void sync_octree_streams(its_batch b) {
@@ -299,6 +286,12 @@ static void sync_octree_streams(its_batch b) {
     }
 }
 
+// Fit every mesh of the batch.  Meshes with at most ITS_TUNE_BATCH_MAX_VERTS
+// unknowns are solved together by the one-launch kernel (graph-captured,
+// replayed on later calls) on the batch's own stream; larger meshes, whose
+// solve on one workgroup would bound the launch, run at the same time on the
+// context stream through the many-mesh solver (one launch per solver pass
+// for all of them).  seconds: wall time of the whole fit.
 void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_stats* out, double* seconds) {
     const int count = (int)b->svos.size();
     ITS_REQUIRE(count > 0, ITS_E_STATE, "empty batch");
@@ -378,49 +371,16 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
         sync_octree_streams(b);
         ITS_HIP(hipGraphLaunch(b->gexec, b->side));
     }
-    // Large meshes: the multi-workgroup solver, whose inner loop is host
-    // driven, from up to kWorkers host threads with a stream each, largest
-    // first.  Each solve only touches its own octree's state, so the results
-    // do not depend on the interleaving.
+    // Large meshes: the many-mesh solver on the context stream, one launch
+    // per solver pass for all of them, bit-identical per mesh to the
+    // single-mesh solver.
     std::vector<its_lscg_stats> big_stats(big.size());
     if (!big.empty()) {
-        std::vector<size_t> order(big.size());
-        for (size_t q = 0; q < big.size(); ++q) {
-            order[q] = q;
-            its_svo s = b->svos[big[q]];
-            AllocStream as(s->ctx->stream);   // solver state built on the octree's own stream
-            (void)fit_get(s);
-            lambda_changing(s);
-            s->lambda64.ensure(s->num_verts);
-        }
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-            return b->svos[big[x]]->num_verts > b->svos[big[y]]->num_verts;
-        });
+        std::vector<its_svo> set(big.size());
+        for (size_t q = 0; q < big.size(); ++q) set[q] = b->svos[big[q]];
         ITS_HIP(hipStreamSynchronize(st));
         sync_octree_streams(b);
-        const int nw = (int)std::min<size_t>(its_batch_s::kWorkers, big.size());
-        for (int w = 0; w < nw; ++w)
-            if (!b->work[w]) ITS_HIP(hipStreamCreateWithFlags(&b->work[w], hipStreamNonBlocking));
-        std::atomic<size_t> next{0};
-        std::vector<std::exception_ptr> errs(nw);
-        const int device = b->ctx->device;
-        auto worker = [&](int w) {
-            try {
-                ITS_HIP(hipSetDevice(device));
-                for (size_t k; (k = next.fetch_add(1)) < order.size();) {
-                    const size_t q = order[k];
-                    its_svo s = b->svos[big[q]];
-                    lscg_solve(s, s->rhs.p, s->lambda64.p, tol, max_iter, warm, &big_stats[q], b->work[w]);
-                }
-            } catch (...) {
-                errs[w] = std::current_exception();
-            }
-        };
-        std::vector<std::thread> threads;
-        for (int w = 0; w < nw; ++w) threads.emplace_back(worker, w);
-        for (std::thread& t : threads) t.join();
-        for (const std::exception_ptr& e : errs)
-            if (e) std::rethrow_exception(e);
+        lscg_solve_many(b->large, set, tol, max_iter, warm, big_stats.data(), st);
     }
     if (ns > 0) ITS_HIP(hipStreamSynchronize(b->side));
     ITS_HIP(hipStreamSynchronize(st));

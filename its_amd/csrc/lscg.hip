@@ -40,11 +40,8 @@ namespace its {
 void fit_release(Fit* f) { delete f; }
 
 namespace {
-constexpr int kT = 256;
-constexpr int kRedBlocks = 512;  // fixed partial count for deterministic sums
-
-// device scalar slots
-enum Scal { S_ABSNEW = 0, S_ALPHA, S_BETA, S_TT, S_RN2, S_THRESH, S_RN2_0, S_ZZ, S_ZZ2, S_COUNT };
+constexpr int kT = kFitT;
+constexpr int kRedBlocks = kFitRed;  // fixed partial count for deterministic sums
 
 // ---------------------------------------------------------------- structure
 __global__ void k_pro_count(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
@@ -210,37 +207,7 @@ __global__ __launch_bounds__(kT) void k_prolong(int64_t b, int64_t e, const uint
     if (UPD && ctl[0]) return;
     int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= e) return;
-    T pi = p[i];
-    if (UPD) {
-        pi = z[i] + (T)scal[S_BETA] * pi;
-        p[i] = pi;
-    }
-    const uint32_t k0 = ptr[i], k1 = ptr[i + 1];
-    const T s = gather_parents(idx, k0, k1, (const T*)g);
-    T w = (k1 > k0) ? (T)1 / (T)(k1 - k0) : (T)0;
-    g[i] = pi + w * s;
-}
-
-__device__ inline double block_sum(double v, double* sh) {
-    v = wave_sum(v);
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double s = 0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
-    __syncthreads();
-    return s;
-}
-
-// Reductions write one partial per block; a one-workgroup kernel that
-// follows sums them (kernel boundaries order the writes, so no fences or
-// atomics are needed) in a fixed order: per-thread strided sums, then a
-// fixed tree.  The total is valid in thread 0.
-__device__ inline double sum_partials(const double* part, unsigned np, int stride, int o, double* sh) {
-    double v = 0;
-    for (unsigned b = threadIdx.x; b < np; b += blockDim.x) v += part[b * stride + o];
-    return block_sum(v, sh);
+    prolong_at<T, UPD>(i, ptr, idx, p, z, g, scal);
 }
 
 // t = D g and the partials of ||t||^2
@@ -248,24 +215,13 @@ __global__ __launch_bounds__(kT) void k_apply_down(int64_t n, const uint32_t* do
                                                   double* part, const int* ctl) {
     if (ctl[0]) return;
     __shared__ double sh[kT / 64];
-    double acc = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float v = g[down[i]];
-        t[i] = v;
-        acc += (double)v * (double)v;
-    }
-    double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    apply_down_block(n, blockIdx.x, gridDim.x, down, g, t, part, sh);
 }
 // alpha = absNew / ||t||^2
 __global__ __launch_bounds__(kT) void k_scalar_alpha(const double* part, unsigned np, double* scal, const int* ctl) {
     if (ctl[0]) return;
     __shared__ double sh[kT / 64];
-    double tt = sum_partials(part, np, 1, 0, sh);
-    if (threadIdx.x == 0) {
-        scal[S_TT] = tt;
-        scal[S_ALPHA] = tt > 0 ? scal[S_ABSNEW] / tt : 0.0;
-    }
+    scalar_alpha_body(part, np, scal, sh);
 }
 
 // x += alpha p; r -= alpha t along each coincidence chain; h = D^T r
@@ -275,16 +231,7 @@ __global__ __launch_bounds__(kT) void k_update_dt(int64_t n, const uint8_t* fine
     if (ctl[0]) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float alpha = do_update ? (float)scal[S_ALPHA] : 0.f;
-    if (do_update) x[i] = fmaf(alpha, p[i], x[i]);
-    if (!finest[i]) { h[i] = 0.f; return; }
-    float s = 0.f;
-    for (uint32_t j = (uint32_t)i; j != ITS_NONE; j = up[j]) {
-        float rj = r[j];
-        if (do_update) { rj = fmaf(-alpha, t[j], rj); r[j] = rj; }
-        s += rj;
-    }
-    h[i] = s;
+    update_dt_at(i, finest, up, x, p, r, t, h, scal, do_update);
 }
 
 // h_c += sum_{y in res(c)} 2^-k(y) h_y for the vertices of one level
@@ -294,87 +241,23 @@ __global__ __launch_bounds__(kT) void k_restrict(int64_t b, int64_t e, const uin
     if (check && ctl[0]) return;
     int64_t c = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= e) return;
-    h[c] += gather_children(ridx, ptr[c], ptr[c + 1], (const T*)h);
+    restrict_at<T>(c, ptr, ridx, h);
 }
 
-// The coarse levels [lc, L) hold few vertices (at most kCoarse).  One
-// workgroup runs all of them with the level values in LDS and a barrier
-// between levels, replacing one launch per level.
-constexpr int kCoarse = 12288;
-constexpr int kCoarseT = 1024;
-// The list offsets of the coarse range are staged in LDS first (one round of
-// loads) when the range holds fewer than kCoarsePtr vertices, so a level
-// waits for its list entries and values only.
-constexpr int kCoarsePtr = 2049;
-__device__ inline bool stage_ptr(const Coarse& C, const uint32_t* ptr, uint32_t* sptr) {
-    const uint32_t base = C.off[C.lc], nptr = C.off[C.L] - base + 1;
-    if (nptr > (uint32_t)kCoarsePtr) return false;
-    for (uint32_t t = threadIdx.x; t < nptr; t += blockDim.x) sptr[t] = ptr[base + t];
-    __syncthreads();
-    return true;
-}
 __global__ __launch_bounds__(kCoarseT) void k_prolong_coarse(Coarse C, const uint32_t* ptr, const uint32_t* idx,
                                                             float* p, const float* z, float* g, const double* scal,
                                                             const int* ctl, int upd) {
     if (upd && ctl[0]) return;
     __shared__ float sg[kCoarse];
     __shared__ uint32_t sptr[kCoarsePtr];
-    const uint32_t base = C.off[C.lc];
-    const bool staged = stage_ptr(C, ptr, sptr);
-    const float beta = upd ? (float)scal[S_BETA] : 0.f;
-    for (int d = C.L - 1; d >= C.lc; --d) {
-        for (uint32_t i = C.off[d] + threadIdx.x; i < C.off[d + 1]; i += blockDim.x) {
-            float pi = p[i];
-            if (upd) { pi = z[i] + beta * pi; p[i] = pi; }
-            const uint32_t k0 = staged ? sptr[i - base] : ptr[i], k1 = staged ? sptr[i + 1 - base] : ptr[i + 1];
-            float s = 0.f;
-            for (uint32_t kb = k0; kb < k1; kb += 8) {
-                uint32_t id[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) id[q] = (kb + q < k1) ? idx[kb + q] : base;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (kb + q < k1) s += sg[id[q] - base];
-            }
-            float w = (k1 > k0) ? 1.f / (float)(k1 - k0) : 0.f;
-            float v = pi + w * s;
-            sg[i - base] = v;
-            g[i] = v;
-        }
-        __syncthreads();
-    }
+    prolong_coarse_body(C, ptr, idx, p, z, g, scal, upd, sg, sptr);
 }
 __global__ __launch_bounds__(kCoarseT) void k_restrict_coarse(Coarse C, const uint32_t* ptr, const uint32_t* ridx,
                                                              float* h, const int* ctl, int check) {
     if (check && ctl[0]) return;
     __shared__ float sh[kCoarse];
     __shared__ uint32_t sptr[kCoarsePtr];
-    const uint32_t base = C.off[C.lc];
-    const bool staged = stage_ptr(C, ptr, sptr);
-    for (int d = C.lc; d < C.L; ++d) {
-        for (uint32_t c = C.off[d] + threadIdx.x; c < C.off[d + 1]; c += blockDim.x) {
-            float s = 0.f;
-            const uint32_t k0 = staged ? sptr[c - base] : ptr[c], k1 = staged ? sptr[c + 1 - base] : ptr[c + 1];
-            for (uint32_t kb = k0; kb < k1; kb += 32) {   // child lists: at most 27 entries
-                uint32_t v[32];
-                float hy[32];
-#pragma unroll
-                for (int q = 0; q < 32; ++q) v[q] = (kb + q < k1) ? ridx[kb + q] : base;   // base: in bounds for both reads
-#pragma unroll
-                for (int q = 0; q < 32; ++q) {
-                    const uint32_t y = v[q] & 0x0FFFFFFFu;
-                    hy[q] = (kb + q < k1) ? ((d == C.lc) ? h[y] : sh[y - base]) : 0.f;
-                }
-#pragma unroll
-                for (int q = 0; q < 32; ++q)
-                    if (kb + q < k1) s += ldexpf(hy[q], -(int)(v[q] >> 28));
-            }
-            float hc = h[c] + s;
-            sh[c - base] = hc;
-            h[c] = hc;
-        }
-        __syncthreads();
-    }
+    restrict_coarse_body(C, ptr, ridx, h, sh, sptr);
 }
 
 // z = M^-1 nr and the partials of nr.nr and nr.z
@@ -382,45 +265,13 @@ __global__ __launch_bounds__(kT) void k_precond(int64_t n, const float* nr, cons
                                                const int* ctl) {
     if (ctl[0]) return;
     __shared__ double sh[kT / 64];
-    double a = 0, c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float v = nr[i];
-        float zi = invd[i] * v;
-        z[i] = zi;
-        a += (double)v * (double)v;
-        c += (double)v * (double)zi;
-    }
-    double sa = block_sum(a, sh);
-    double sc = block_sum(c, sh);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sa;
-        part[2 * blockIdx.x + 1] = sc;
-    }
+    precond_block(n, blockIdx.x, gridDim.x, nr, invd, z, part, sh);
 }
 // rn2, absNew', the stopping test of Eigen's loop, beta
 __global__ __launch_bounds__(kT) void k_scalar_beta(const double* part, unsigned np, double* scal, int* ctl, int init) {
     if (ctl[0]) return;
     __shared__ double sh[kT / 64];
-    double rn2 = sum_partials(part, np, 2, 0, sh);
-    double zz = sum_partials(part, np, 2, 1, sh);
-    if (threadIdx.x == 0) {
-        scal[S_RN2] = rn2;
-        if (init) {
-            scal[S_RN2_0] = rn2;
-            scal[S_ABSNEW] = zz;
-            scal[S_BETA] = 0.0;
-            if (rn2 < scal[S_THRESH] || zz == 0.0) ctl[0] = 1;
-        } else {
-            ctl[1] += 1;
-            if (rn2 < scal[S_THRESH] || ctl[1] >= ctl[3]) {
-                ctl[0] = 1;
-            } else {
-                double absOld = scal[S_ABSNEW];
-                scal[S_ABSNEW] = zz;
-                scal[S_BETA] = zz / absOld;
-            }
-        }
-    }
+    scalar_beta_body(part, np, scal, ctl, init, sh);
 }
 
 // binary64 helpers for the refinement loop
@@ -430,11 +281,7 @@ __global__ void k_apply_down64(int64_t n, const uint32_t* down, const double* g,
 }
 __global__ void k_dt64(int64_t n, const uint8_t* finest, const uint32_t* up, const double* r, double* h) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (!finest[i]) { h[i] = 0.0; return; }
-    double s = 0.0;
-    for (uint32_t j = (uint32_t)i; j != ITS_NONE; j = up[j]) s += r[j];
-    h[i] = s;
+    if (i < n) dt64_at(i, finest, up, r, h);
 }
 __global__ void k_sub64(int64_t n, const double* b, const double* a, double* r) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -450,11 +297,7 @@ __global__ void k_axpy_f32_to64(int64_t n, const float* x, double* y) {
 }
 __global__ __launch_bounds__(kT) void k_dot64(int64_t n, const double* a, const double* b, double* part) {
     __shared__ double sh[kT / 64];
-    double acc = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        acc += a[i] * b[i];
-    double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    dot64_block(n, blockIdx.x, gridDim.x, a, b, part, sh);
 }
 __global__ __launch_bounds__(kT) void k_sum_partials(const double* part, unsigned np, double* out) {
     __shared__ double sh[kT / 64];
@@ -464,7 +307,7 @@ __global__ __launch_bounds__(kT) void k_sum_partials(const double* part, unsigne
 __global__ void k_set_thresh(double* scal, double rel2) { scal[S_THRESH] = rel2 * scal[S_RN2_0]; }
 
 // ---------------------------------------------------------------- host side
-unsigned red_blocks(int64_t n) { return (unsigned)std::min<int64_t>(kRedBlocks, blocks_for(n, kT)); }
+unsigned red_blocks(int64_t n) { return fit_red_blocks(n); }
 
 }  // namespace
 

@@ -1,11 +1,14 @@
 """Many meshes per launch (GPU): per-mesh results equal the single-mesh path.
 
 No reference counterpart (the reference fits one model per process).  The
-batched fit runs, per mesh, the algorithm and stopping rule of
+one-launch fit runs, per mesh, the algorithm and stopping rule of
 its_lscg_solve; only the summation order of its binary64 reductions differs,
-so lambda agrees to the solver tolerance and f to 1e-5.  The batched
-evaluation uses the arithmetic of the per-thread walk, so f is bit-identical
-to its_bspline_eval_device with one query per thread.
+so lambda agrees to the solver tolerance and f to 1e-5.  Meshes above the
+one-launch cap go through the many-mesh solver, whose per-mesh blocks,
+reductions and arithmetic are those of its_lscg_solve: their coefficients and
+iteration counts are bit-identical.  The batched evaluation uses the
+arithmetic of the per-thread walk, so f is bit-identical to
+its_bspline_eval_device with one query per thread.
 """
 import numpy as np
 import pytest
@@ -71,6 +74,10 @@ def test_batch_fit_and_eval_match_single(ctx, cap):
         lam_b = g.get_lambda()
         scale = max(np.abs(lam_s).max(), 1e-30)
         assert np.abs(lam_b - lam_s).max() <= 1e-5 * scale, c
+        if g.num_verts > cap:   # the many-mesh solver
+            assert np.array_equal(lam_b, lam_s), c
+            for k in ("iterations", "refinements", "normal_rel", "residual_abs", "initial_normal_rel"):
+                assert st_b[k] == st_s[k], (c, k, st_b[k], st_s[k])
         if counts[m]:
             q = Q[offs[m]:offs[m + 1]]
             f_single = g.eval(q, c.origin + c.width, _capi.EVAL_SUPPORT, 0.5)
