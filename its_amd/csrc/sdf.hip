@@ -722,28 +722,41 @@ __device__ inline void wave_argmin(double& d2, int32_t& id) {
     }
 }
 
-// nearest triangle of a leaf, one triangle per lane; a triangle whose
-// bounding sphere is farther than the best so far is skipped
-__device__ inline void leaf_nearest(const TreeView& t, int h, int64_t k, int lane, const double p[3], float pm,
-                                    double& best, int32_t& bt) {
-    const int64_t i = node_first(h, k) + lane;
+// Values that are equal in every lane (the point of the wavefront, the best
+// distance after the butterfly) are moved to scalar registers, which the
+// compiler cannot infer from threadIdx.x >> 6 or from shuffles.
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double wave_uniform(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// false when the bounding sphere of sorted triangle i is farther than the
+// best distance so far
+__device__ inline bool sphere_within(const TreeView& t, int64_t i, const double p[3], float pm, double best) {
+    const float4 S = t.sph[i];
+    const float dx = (float)p[0] - S.x, dy = (float)p[1] - S.y, dz = (float)p[2] - S.z;
+    const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
+    const float lb = sqrtf(dx * dx + dy * dy + dz * dz) - S.w - m;
+    return !(lb > 0.f && (double)lb * (double)lb > best);
+}
+// binary64 distance of sorted triangle i (when take), merged over the
+// wavefront into (best, bt)
+__device__ inline void tri_nearest(const TreeView& t, int64_t i, bool take, const double p[3], double& best,
+                                   int32_t& bt) {
     double d2 = INFINITY;
     int32_t id = 0x7fffffff;
-    if (i < node_last(t.s, h, k)) {
-        const float4 S = t.sph[i];
-        const float dx = (float)p[0] - S.x, dy = (float)p[1] - S.y, dz = (float)p[2] - S.z;
-        const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
-        const float lb = sqrtf(dx * dx + dy * dy + dz * dz) - S.w - m;
-        if (!(lb > 0.f && (double)lb * (double)lb > best)) {
-            const double* T = t.tris + 9 * i;
-            d2 = point_tri_dist2(p, T, T + 3, T + 6);
-            id = t.tid[i];
-            // a degenerate triangle can give 0/0; like a plain "d2 < best"
-            // scan, a NaN never wins (and must not reach the butterfly)
-            if (!(d2 <= INFINITY)) {
-                d2 = INFINITY;
-                id = 0x7fffffff;
-            }
+    if (take) {
+        const double* T = t.tris + 9 * i;
+        d2 = point_tri_dist2(p, T, T + 3, T + 6);
+        id = t.tid[i];
+        // a degenerate triangle can give 0/0; like a plain "d2 < best"
+        // scan, a NaN never wins (and must not reach the butterfly)
+        if (!(d2 <= INFINITY)) {
+            d2 = INFINITY;
+            id = 0x7fffffff;
         }
     }
     wave_argmin(d2, id);
@@ -751,6 +764,16 @@ __device__ inline void leaf_nearest(const TreeView& t, int h, int64_t k, int lan
         best = d2;
         bt = id;
     }
+    best = wave_uniform(best);
+    bt = wave_uniform(bt);
+}
+// nearest triangle of a leaf, one triangle per lane; a triangle whose
+// bounding sphere is farther than the best so far is skipped
+__device__ inline void leaf_nearest(const TreeView& t, int h, int64_t k, int lane, const double p[3], float pm,
+                                    double& best, int32_t& bt) {
+    const int64_t i = node_first(h, k) + lane;
+    const bool take = i < node_last(t.s, h, k) && sphere_within(t, i, p, pm, best);
+    tri_nearest(t, i, take, p, best, bt);
 }
 
 // true when no point of the box can be within the best distance found so
@@ -776,6 +799,18 @@ __device__ inline void lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The frontier walks start at the lowest level of at most 128 nodes (not
+// below the leaves), testing all its nodes at once, two per lane, instead of
+// descending from the root one level per round.  A box holds its children's
+// boxes, so a node that the walk from the root would have dropped with an
+// ancestor only makes the frontier larger, never changes the result.
+constexpr int kStartNodes = 128;
+__device__ inline int start_level(const Shape& s, int leaf) {
+    int h = s.H;
+    while (h > leaf && s.count[h - 1] <= kStartNodes) --h;
+    return h;
 }
 
 __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
@@ -817,36 +852,61 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
                                                  int32_t* tri_out) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kFront];   // frontier node indices within their level
     __shared__ float flb[kWavesPerBlock][kFront];        // box lower bounds of the leaf-level frontier
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ uint32_t cq[kWavesPerBlock][128];         // candidate triangles (sorted positions)
+    const int lane = threadIdx.x & 63, w = wave_uniform((int)(threadIdx.x >> 6));
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
+    const int start = start_level(t.s, leaf), start_count = t.s.count[start];
     for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
         const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
         const float pm = fmaxf(fabsf(pf[0]), fmaxf(fabsf(pf[1]), fabsf(pf[2])));
         double best = INFINITY;
         int32_t bt = 0x7fffffff;
+        // the lower bound of c among the sorted codes lies within its prefix's range
         const uint32_t c = quant_code(t.q, p);
-        int64_t a = 0, b = t.s.T;
+        const uint32_t C = c >> (30 - t.cell_bits);
+        int64_t a = t.cell_lb[C], b = t.cell_lb[C + 1];
         while (a < b) {
             const int64_t m = (a + b) >> 1;
             if (t.code[m] < c) a = m + 1; else b = m;
         }
         const int64_t seed = std::min<int64_t>(a, t.s.T - 1) / ((int64_t)kBucket << leaf);
         leaf_nearest(t, leaf, seed, lane, p, pm, best, bt);
-        // frontier: the root, unless its box is beyond the seed's distance
-        const NodeRec root = load_node(t, t.s.off[t.s.H]);
-        const float root_lb = box_lb(pf, pm, root.lo, root.hi);
-        int n = lb_beyond(root_lb, best) ? 0 : 1, cur = 0;
-        lds_fence();
-        if (lane == 0) {
-            fk[0][w][0] = 0u;
-            flb[w][0] = root_lb;
+        // frontier: the nodes of the start level whose boxes are within the
+        // seed's distance
+        int n = 0, cur = 0;
+        {
+            float lb0 = 0.f, lb1 = 0.f;
+            bool keep0 = false, keep1 = false;
+            if (lane < start_count) {
+                const NodeRec r0 = load_node(t, t.s.off[start] + lane);
+                lb0 = box_lb(pf, pm, r0.lo, r0.hi);
+                keep0 = !lb_beyond(lb0, best);
+            }
+            if (lane + 64 < start_count) {
+                const NodeRec r1 = load_node(t, t.s.off[start] + lane + 64);
+                lb1 = box_lb(pf, pm, r1.lo, r1.hi);
+                keep1 = !lb_beyond(lb1, best);
+            }
+            const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+            lds_fence();
+            if (keep0) {
+                const int pos = __popcll(m0 & below);
+                fk[0][w][pos] = (uint32_t)lane;
+                flb[w][pos] = lb0;
+            }
+            if (keep1) {
+                const int pos = __popcll(m0) + __popcll(m1 & below);
+                fk[0][w][pos] = (uint32_t)lane + 64u;
+                flb[w][pos] = lb1;
+            }
+            n = __popcll(m0) + __popcll(m1);
+            lds_fence();
         }
-        lds_fence();
         bool overflow = false;
-        for (int h = t.s.H; h > leaf && n > 0; --h) {
+        for (int h = start; h > leaf && n > 0; --h) {
             const int64_t coff = t.s.off[h - 1];
             const int ccount = t.s.count[h - 1];
             const bool to_leaf = (h - 1 == leaf);
@@ -892,11 +952,35 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
         if (overflow) {
             dist_dfs(t, leaf, seed, lane, p, pf, pm, best, bt);
         } else {
+            // Candidates: the triangles of the surviving leaves that pass
+            // the sphere test are queued and their binary64 distances taken
+            // 64 at a time, so the wavefront runs the exact distance once per
+            // 64 candidates rather than once per leaf.  A test against an
+            // older, larger best only queues more, so the minimum is that of
+            // the full search.
+            int qn = 0;   // queued candidates, below 64 between leaves
             for (int j = 0; j < n; ++j) {
                 const int64_t k = fk[cur][w][j];
                 if (k == seed || lb_beyond(flb[w][j], best)) continue;
-                leaf_nearest(t, leaf, k, lane, p, pm, best, bt);
+                const int64_t ti = node_first(leaf, k) + lane;
+                const bool pass = ti < node_last(t.s, leaf, k) && sphere_within(t, ti, p, pm, best);
+                const uint64_t mp = __ballot(pass);
+                if (pass) cq[w][qn + __popcll(mp & below)] = (uint32_t)ti;
+                qn += __popcll(mp);
+                if (qn >= 64) {
+                    lds_fence();
+                    tri_nearest(t, cq[w][lane], true, p, best, bt);
+                    qn -= 64;
+                    const uint32_t rest = lane < qn ? cq[w][64 + lane] : 0u;
+                    lds_fence();
+                    if (lane < qn) cq[w][lane] = rest;
+                }
             }
+            if (qn > 0) {
+                lds_fence();
+                tri_nearest(t, lane < qn ? cq[w][lane] : 0u, lane < qn, p, best, bt);
+            }
+            lds_fence();
         }
         if (lane == 0) {
             out[i] = sqrt(best);
@@ -999,6 +1083,20 @@ __device__ inline int o3d_sign(const double* a, const double* b, const double* c
     return det > bound ? 1 : (-det > bound ? -1 : 0);
 }
 
+// false when the +z ray from p surely misses sorted triangle j: its bounding
+// sphere, which holds the triangle with a margin for the binary32 rounding
+// here, lies wholly beside the ray or wholly below p.  Such a triangle is
+// rejected without reading its binary64 vertices; that p lies outside its
+// projection is then certain, so the crossing count is unchanged.
+__device__ inline bool sphere_on_ray(const TreeView& t, int64_t j, const double p[3], float pm) {
+    const float4 S = t.sph[j];
+    const float dx = (float)p[0] - S.x, dy = (float)p[1] - S.y, dz = (float)p[2] - S.z;
+    const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
+    const float beside = sqrtf(dx * dx + dy * dy) - S.w - m;
+    const float below = dz - S.w - m;
+    return !(beside > 0.f) && !(below > 0.f);
+}
+
 // Signed crossing of the +z ray from p with one triangle: +1 / -1 (the sign
 // of its xy orientation) when the ray crosses it above p, else 0; unsure is
 // set when an orientation that decides this is not certain.
@@ -1052,18 +1150,29 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
+    const int start = start_level(t.s, leaf), start_count = t.s.count[start];
     for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
         bool unsure = !(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]));
         int n = 0, cur = 0, nl = 0;
-        if (!unsure) {
-            const NodeRec root = load_node(t, t.s.off[t.s.H]);
-            n = box_on_ray(p, root.lo, root.hi) ? 1 : 0;
+        {   // the nodes of the start level whose boxes meet the ray
+            bool keep0 = false, keep1 = false;
+            if (!unsure && lane < start_count) {
+                const NodeRec r0 = load_node(t, t.s.off[start] + lane);
+                keep0 = box_on_ray(p, r0.lo, r0.hi);
+            }
+            if (!unsure && lane + 64 < start_count) {
+                const NodeRec r1 = load_node(t, t.s.off[start] + lane + 64);
+                keep1 = box_on_ray(p, r1.lo, r1.hi);
+            }
+            const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+            lds_fence();
+            if (keep0) fk[0][w][__popcll(m0 & below)] = (uint32_t)lane;
+            if (keep1) fk[0][w][__popcll(m0) + __popcll(m1 & below)] = (uint32_t)lane + 64u;
+            n = __popcll(m0) + __popcll(m1);
+            lds_fence();
         }
-        lds_fence();
-        if (lane == 0) fk[0][w][0] = 0u;
-        lds_fence();
-        for (int h = t.s.H; n > 0; --h) {
+        for (int h = start; n > 0; --h) {
             if (h <= leaf) {   // every node of this level is a leaf
                 if (nl + n > kRayLeaves) {
                     unsure = true;
@@ -1108,12 +1217,15 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
         }
         int cross = 0;
         bool lane_unsure = false;
-        if (!unsure)
+        if (!unsure) {
+            const float pm = fmaxf(fabsf((float)p[0]), fmaxf(fabsf((float)p[1]), fabsf((float)p[2])));
             for (int l = 0; l < nl; ++l) {
                 const int64_t k = leaves[w][l];
                 const int64_t j = node_first(leaf, k) + lane;
-                if (j < node_last(t.s, leaf, k)) cross += tri_ray(t.tris + 9 * j, p, lane_unsure);
+                if (j < node_last(t.s, leaf, k) && sphere_on_ray(t, j, p, pm))
+                    cross += tri_ray(t.tris + 9 * j, p, lane_unsure);
             }
+        }
         unsure = unsure || __ballot(lane_unsure) != 0ull;
         const int wn = wave_sum(cross);
         if (lane == 0) {
