@@ -36,10 +36,12 @@
 //     has exactly two half edges of opposite directions, the mesh is an
 //     oriented closed surface, and its winding number at p is the signed
 //     number of crossings of any ray from p (self-intersecting and nested
-//     surfaces included).  The +z ray is tested against the triangles whose
-//     boxes it meets: p is inside the xy projection of a triangle when its
-//     three edge orientations agree, and the crossing lies above p when the
-//     orientation of the tetrahedron (a, b, c, p) has that same sign.  Each
+//     surfaces included).  Each point takes the axis direction (+-x, +-y,
+//     +-z) that leaves the mesh's box soonest; the ray is tested against
+//     the triangles whose boxes it meets: p is inside the projection of a
+//     triangle across the ray when its three edge orientations agree, and
+//     the crossing lies ahead of p when the orientation of the tetrahedron
+//     (a, b, c, p) has the sign of the triangle's normal along the ray.  Each
 //     orientation is a binary64 determinant whose sign is taken only when it
 //     exceeds the rounding bound of its terms (the static filters of
 //     Shewchuk, Adaptive Precision Floating-Point Arithmetic and Fast Robust
@@ -1083,87 +1085,125 @@ __device__ inline int o3d_sign(const double* a, const double* b, const double* c
     return det > bound ? 1 : (-det > bound ? -1 : 0);
 }
 
-// false when the +z ray from p surely misses sorted triangle j: its bounding
-// sphere, which holds the triangle with a margin for the binary32 rounding
-// here, lies wholly beside the ray or wholly below p.  Such a triangle is
-// rejected without reading its binary64 vertices; that p lies outside its
-// projection is then certain, so the crossing count is unchanged.
-__device__ inline bool sphere_on_ray(const TreeView& t, int64_t j, const double p[3], float pm) {
-    const float4 S = t.sph[j];
-    const float dx = (float)p[0] - S.x, dy = (float)p[1] - S.y, dz = (float)p[2] - S.z;
-    const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
-    const float beside = sqrtf(dx * dx + dy * dy) - S.w - m;
-    const float below = dz - S.w - m;
-    return !(beside > 0.f) && !(below > 0.f);
+// An axis ray from p: along axis w in direction sg (+1 or -1), with u, v the
+// next two axes in cyclic order, so (u, v, w) is oriented like (x, y, z) and
+// the (u, v) orientation of a triangle has the sign of its normal's w
+// component.  Any ray gives the winding number; each point takes the axis
+// direction that leaves the mesh's box soonest, which meets the fewest nodes.
+struct Ray {
+    int u, v, w, sg;
+};
+__device__ inline double sel3(const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); }
+__device__ inline float sel3(float4 x, int i) { return i == 0 ? x.x : (i == 1 ? x.y : x.z); }
+__device__ inline Ray choose_ray(const double p[3], float4 lo, float4 hi) {
+    Ray R{0, 1, 2, 1};
+    float len = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        const float pa = (float)sel3(p, a);
+        const float ahead = sel3(hi, a) - pa, behind = pa - sel3(lo, a);
+        if (ahead < len) {
+            len = ahead;
+            R = Ray{(a + 1) % 3, (a + 2) % 3, a, 1};
+        }
+        if (behind < len) {
+            len = behind;
+            R = Ray{(a + 1) % 3, (a + 2) % 3, a, -1};
+        }
+    }
+    return R;
 }
 
-// Signed crossing of the +z ray from p with one triangle: +1 / -1 (the sign
-// of its xy orientation) when the ray crosses it above p, else 0; unsure is
-// set when an orientation that decides this is not certain.
-__device__ inline int tri_ray(const double* T, const double p[3], bool& unsure) {
+// false when the ray surely misses sorted triangle j: its bounding sphere,
+// which holds the triangle with a margin for the binary32 rounding here,
+// lies wholly beside the ray or wholly behind p.  Such a triangle is
+// rejected without reading its binary64 vertices; that p lies outside its
+// projection is then certain, so the crossing count is unchanged.
+__device__ inline bool sphere_on_ray(const TreeView& t, int64_t j, const double p[3], float pm, const Ray& R) {
+    const float4 S = t.sph[j];
+    const float du = (float)sel3(p, R.u) - sel3(S, R.u), dv = (float)sel3(p, R.v) - sel3(S, R.v);
+    const float dw = (float)sel3(p, R.w) - sel3(S, R.w);
+    const float m = bound_margin(pm + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))) + S.w);
+    const float beside = sqrtf(du * du + dv * dv) - S.w - m;
+    const float behind = (R.sg > 0 ? dw : -dw) - S.w - m;
+    return !(beside > 0.f) && !(behind > 0.f);
+}
+
+// Signed crossing of the ray from p with one triangle: +1 / -1 (the sign of
+// n . d, its normal against the ray direction) when the ray crosses it ahead
+// of p, else 0; unsure is set when an orientation that decides this is not
+// certain.
+__device__ inline int tri_ray(const double* T, const double p[3], const Ray& R, bool& unsure) {
     const double* a = T;
     const double* b = T + 3;
     const double* c = T + 6;
-    // outside the xy bounding rectangle, or wholly below p: no crossing
-    if (p[0] < fmin(a[0], fmin(b[0], c[0])) || p[0] > fmax(a[0], fmax(b[0], c[0])) ||
-        p[1] < fmin(a[1], fmin(b[1], c[1])) || p[1] > fmax(a[1], fmax(b[1], c[1])) ||
-        fmax(a[2], fmax(b[2], c[2])) < p[2])
+    const double pu = sel3(p, R.u), pv = sel3(p, R.v), pw = sel3(p, R.w);
+    const double au = a[R.u], av = a[R.v], aw = a[R.w];
+    const double bu = b[R.u], bv = b[R.v], bw = b[R.w];
+    const double cu = c[R.u], cv = c[R.v], cw = c[R.w];
+    const double wmin = fmin(aw, fmin(bw, cw)), wmax = fmax(aw, fmax(bw, cw));
+    // outside the (u, v) bounding rectangle, or wholly behind p: no crossing
+    if (pu < fmin(au, fmin(bu, cu)) || pu > fmax(au, fmax(bu, cu)) || pv < fmin(av, fmin(bv, cv)) ||
+        pv > fmax(av, fmax(bv, cv)) || (R.sg > 0 ? wmax < pw : wmin > pw))
         return 0;
-    const int s1 = o2d_sign(a[0], a[1], b[0], b[1], p[0], p[1]);
-    const int s2 = o2d_sign(b[0], b[1], c[0], c[1], p[0], p[1]);
-    const int s3 = o2d_sign(c[0], c[1], a[0], a[1], p[0], p[1]);
+    const int s1 = o2d_sign(au, av, bu, bv, pu, pv);
+    const int s2 = o2d_sign(bu, bv, cu, cv, pu, pv);
+    const int s3 = o2d_sign(cu, cv, au, av, pu, pv);
     const bool pos = s1 > 0 || s2 > 0 || s3 > 0, neg = s1 < 0 || s2 < 0 || s3 < 0;
     if (pos && neg) return 0;   // two certain signs disagree: outside the projection
     if (s1 == 0 || s2 == 0 || s3 == 0) {
         unsure = true;
         return 0;
     }
-    // inside the projection; s1 is the xy orientation of (a, b, c).  The
-    // ray meets the plane at t = ((a - p) . n) / n_z, n = (b - a) x (c - a),
-    // and (a - p) . n is the orientation of (a, b, c, p): above p when it
-    // has the sign of n_z
-    if (fmin(a[2], fmin(b[2], c[2])) > p[2]) return s1;
+    // inside the projection; s1 is the (u, v) orientation of (a, b, c), the
+    // sign of n_w, so n . d has the sign dir.  The ray meets the plane at
+    // t = ((a - p) . n) / (n . d), n = (b - a) x (c - a), and (a - p) . n is
+    // the orientation of (a, b, c, p): ahead of p when it has the sign dir
+    const int dir = R.sg > 0 ? s1 : -s1;
+    if (R.sg > 0 ? wmin > pw : wmax < pw) return dir;   // wholly ahead
     const int s4 = o3d_sign(a, b, c, p);
     if (s4 == 0) {
         unsure = true;
         return 0;
     }
-    return s4 == s1 ? s1 : 0;
+    return s4 == dir ? dir : 0;
 }
 
-// Winding numbers of a closed mesh by the +z ray (see the file comment):
+// Winding numbers of a closed mesh by an axis ray (see the file comment):
 // the wavefront walks the tree level by level, keeping the nodes whose
 // boxes meet the ray, then tests the triangles of the leaves it reaches,
 // one per lane.  decided[i] = 1 and the sign is applied when every
 // orientation was certain and the lists fitted; otherwise decided[i] = 0
 // and the fan walk takes the point.
 constexpr int kRayFront = 128, kRayLeaves = 64;
-__device__ inline bool box_on_ray(const double p[3], float4 lo, float4 hi) {
-    return p[0] >= (double)lo.x && p[0] <= (double)hi.x && p[1] >= (double)lo.y && p[1] <= (double)hi.y &&
-           p[2] <= (double)hi.z;
+__device__ inline bool box_on_ray(const double p[3], float4 lo, float4 hi, const Ray& R) {
+    const double pu = sel3(p, R.u), pv = sel3(p, R.v), pw = sel3(p, R.w);
+    return pu >= (double)sel3(lo, R.u) && pu <= (double)sel3(hi, R.u) && pv >= (double)sel3(lo, R.v) &&
+           pv <= (double)sel3(hi, R.v) && (R.sg > 0 ? pw <= (double)sel3(hi, R.w) : pw >= (double)sel3(lo, R.w));
 }
 __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, int64_t N, double* out,
                                                uint8_t* decided) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kRayFront];
     __shared__ uint32_t leaves[kWavesPerBlock][kRayLeaves];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = wave_uniform((int)(threadIdx.x >> 6));
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
     const int start = start_level(t.s, leaf), start_count = t.s.count[start];
+    const NodeRec root = load_node(t, t.s.off[t.s.H]);
     for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
         bool unsure = !(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]));
+        const Ray R = choose_ray(p, root.lo, root.hi);
         int n = 0, cur = 0, nl = 0;
         {   // the nodes of the start level whose boxes meet the ray
             bool keep0 = false, keep1 = false;
             if (!unsure && lane < start_count) {
                 const NodeRec r0 = load_node(t, t.s.off[start] + lane);
-                keep0 = box_on_ray(p, r0.lo, r0.hi);
+                keep0 = box_on_ray(p, r0.lo, r0.hi, R);
             }
             if (!unsure && lane + 64 < start_count) {
                 const NodeRec r1 = load_node(t, t.s.off[start] + lane + 64);
-                keep1 = box_on_ray(p, r1.lo, r1.hi);
+                keep1 = box_on_ray(p, r1.lo, r1.hi, R);
             }
             const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
             lds_fence();
@@ -1193,10 +1233,10 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
                 if (j < n) {
                     c0 = 2u * fk[cur][w][j];
                     const NodeRec r0 = load_node(t, (int)(coff + c0));
-                    keep0 = box_on_ray(p, r0.lo, r0.hi);
+                    keep0 = box_on_ray(p, r0.lo, r0.hi, R);
                     if ((int)c0 + 1 < cnext) {
                         const NodeRec r1 = load_node(t, (int)(coff + c0 + 1));
-                        keep1 = box_on_ray(p, r1.lo, r1.hi);
+                        keep1 = box_on_ray(p, r1.lo, r1.hi, R);
                     }
                 }
                 const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
@@ -1222,8 +1262,8 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
             for (int l = 0; l < nl; ++l) {
                 const int64_t k = leaves[w][l];
                 const int64_t j = node_first(leaf, k) + lane;
-                if (j < node_last(t.s, leaf, k) && sphere_on_ray(t, j, p, pm))
-                    cross += tri_ray(t.tris + 9 * j, p, lane_unsure);
+                if (j < node_last(t.s, leaf, k) && sphere_on_ray(t, j, p, pm, R))
+                    cross += tri_ray(t.tris + 9 * j, p, R, lane_unsure);
             }
         }
         unsure = unsure || __ballot(lane_unsure) != 0ull;

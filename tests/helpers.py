@@ -132,22 +132,25 @@ def bbox_known_answers():
 
 
 
-def exact_winding_number(V, F, p, axis=0):
+def exact_winding_number(V, F, p, axis=0, sign=1):
     """Winding number of a closed, consistently oriented triangle mesh at p,
-    exactly: the signed count of crossings of the ray from p along +axis,
-    every orientation evaluated in rational arithmetic (fractions) from the
-    binary64 coordinates.  None when the ray meets an edge or p lies on a
-    triangle's plane where it crosses (then another axis decides)."""
+    exactly: the signed count of crossings of the ray from p along axis in
+    direction sign (+1 or -1), every orientation evaluated in rational
+    arithmetic (fractions) from the binary64 coordinates.  A crossing counts
+    the sign of the triangle's normal along the ray.  None when the ray meets
+    an edge or p lies on a triangle's plane where it crosses (then another
+    ray decides)."""
     from fractions import Fraction as Q
 
     V = np.asarray(V, np.float64)
     u, v = (axis + 1) % 3, (axis + 2) % 3   # (u, v, axis) is a right-handed frame
     P = [Q(float(x)) for x in p]
     T = V[np.asarray(F)]
-    # triangles whose (u, v) box holds p and that reach beyond p along axis
+    # triangles whose (u, v) box holds p and that reach beyond p along the ray
     lo, hi = T.min(1), T.max(1)
+    ahead = (hi[:, axis] >= p[axis]) if sign > 0 else (lo[:, axis] <= p[axis])
     cand = np.nonzero((lo[:, u] <= p[u]) & (hi[:, u] >= p[u]) & (lo[:, v] <= p[v]) & (hi[:, v] >= p[v]) &
-                      (hi[:, axis] >= p[axis]))[0]
+                      ahead)[0]
 
     def o2(a, b):
         return (a[u] - P[u]) * (b[v] - P[v]) - (a[v] - P[v]) * (b[u] - P[u])
@@ -160,7 +163,8 @@ def exact_winding_number(V, F, p, axis=0):
             continue
         if any(x == 0 for x in s):
             return None
-        sg = 1 if s[0] > 0 else -1
+        # s[0]: the sign of the normal's axis component; along the ray: sg
+        sg = (1 if s[0] > 0 else -1) * (1 if sign > 0 else -1)
         ad = [a[k] - P[k] for k in range(3)]
         bd = [b[k] - P[k] for k in range(3)]
         cd = [c[k] - P[k] for k in range(3)]
@@ -168,6 +172,6 @@ def exact_winding_number(V, F, p, axis=0):
         det = sum(ad[k] * n[k] for k in range(3))
         if det == 0:
             return None
-        if (det > 0) == (sg > 0):
+        if (det > 0) == (sg > 0):   # the plane is crossed ahead of p
             w += sg
     return w

@@ -132,8 +132,8 @@ def test_sdf_ray_signs(ctx, name):
     signed distances are identical bit for bit.  Within 1e-9 of it (here
     edge midpoints, rounded off their edges by about 1e-17) the binary32 fan
     sum can miss the side; wherever the two differ the ray's sign is checked
-    against the exact winding number (rational arithmetic along another
-    axis, tests/helpers.py), negative where it is at least 1."""
+    against the exact winding number (rational arithmetic along an axis ray,
+    tests/helpers.py), negative where it is at least 1."""
     from its_amd import _capi, meshes
     if name == "icosphere":
         V, F = meshes.noisy_icosphere(seed=3, level=4)
@@ -168,8 +168,8 @@ def test_sdf_ray_signs(ctx, name):
     w = []
     for i in sel:
         wi = None
-        for axis in (0, 1, 2):
-            wi = exact_winding_number(V, F, P[i], axis)
+        for axis, sign in ((0, 1), (1, 1), (2, 1), (0, -1), (1, -1), (2, -1)):
+            wi = exact_winding_number(V, F, P[i], axis, sign)
             if wi is not None:
                 break
         w.append(wi)
