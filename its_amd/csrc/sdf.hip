@@ -254,7 +254,6 @@ struct TreeView {
     const float* fan;        // 6 per fan edge: a - centre, b - centre (centre of its node's box)
     const uint32_t* cell_lb; // per cell_bits-bit code prefix C: first sorted triangle with code >= C's range
     int cell_bits;
-    int closed;              // every undirected edge: exactly two half edges, opposite directions
 };
 
 // -------------------------------------------------------------- build -----
@@ -363,30 +362,33 @@ template <class Sink>
 __device__ void edge_walk(const uint64_t* keys, const uint32_t* vals, int64_t i, int64_t M, const Shape& s, Sink&& sink) {
     int64_t end = i + 1;
     while (end < M && keys[end] == keys[i]) ++end;
+    static_assert((kBucket & (kBucket - 1)) == 0, "node spans are powers of two");
+    const int bucket_shift = __builtin_ctz(kBucket);
     for (int h = 1; h <= s.H; ++h) {
-        const int64_t span = (int64_t)kBucket << h;
-        const int64_t k0 = (int64_t)(vals[i] >> 1) / span;
+        // node of level h holding sorted triangle t: t / (kBucket << h)
+        const int shift = bucket_shift + h;
+        const uint32_t k0 = (vals[i] >> 1) >> shift;
         bool same = true;
         int tot = 0;
         for (int64_t j = i; j < end; ++j) {
             const uint32_t v = vals[j];
-            if ((int64_t)(v >> 1) / span != k0) same = false;
+            if (((v >> 1) >> shift) != k0) same = false;
             tot += (v & 1u) ? -1 : 1;
         }
         if (same) {
             if (tot == 0) return;
-            sink(s.off[h] + k0, h, k0, tot);
+            sink(s.off[h] + (int64_t)k0, h, (int64_t)k0, tot);
             continue;
         }
         for (int64_t j = i; j < end; ++j) {
-            const int64_t kj = (int64_t)(vals[j] >> 1) / span;
+            const uint32_t kj = (vals[j] >> 1) >> shift;
             bool seen = false;
-            for (int64_t jj = i; jj < j && !seen; ++jj) seen = ((int64_t)(vals[jj] >> 1) / span == kj);
+            for (int64_t jj = i; jj < j && !seen; ++jj) seen = (((vals[jj] >> 1) >> shift) == kj);
             if (seen) continue;
             int coef = 0;
             for (int64_t jj = j; jj < end; ++jj)
-                if ((int64_t)(vals[jj] >> 1) / span == kj) coef += (vals[jj] & 1u) ? -1 : 1;
-            if (coef) sink(s.off[h] + kj, h, kj, coef);
+                if (((vals[jj] >> 1) >> shift) == kj) coef += (vals[jj] & 1u) ? -1 : 1;
+            if (coef) sink(s.off[h] + (int64_t)kj, h, (int64_t)kj, coef);
         }
     }
 }
@@ -546,9 +548,16 @@ struct TriTree {
     DBuf<int32_t> tid;
     DBuf<uint32_t> code, cell_lb;
     int cell_bits = 0;
-    bool closed = false;
+    bool closed = false;   // every undirected edge: exactly two half edges, opposite directions
+    // half edges sorted by undirected edge (for the fans) and the open flag
+    DBuf<uint64_t> hkeys;
+    DBuf<uint32_t> hvals;
+    DBuf<int> is_open;
+    int64_t M = 0;
+    uint64_t none = 0;
+    int vb = 0;
     TreeView view() const {
-        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p, cell_lb.p, cell_bits, closed ? 1 : 0};
+        return TreeView{s, q, tris.p, sph.p, tid.p, code.p, node.p, fan.p, cell_lb.p, cell_bits};
     }
 };
 
@@ -628,41 +637,52 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
         ITS_LAUNCH_CHECK();
     }
 
-    // boundary fans: half edges grouped by undirected edge
+    // half edges grouped by undirected edge: whether the mesh is closed, and
+    // the boundaries of the fans
     const int64_t M = 3 * nF;
     const int vb = bits_for((uint64_t)(nV - 1));
     const uint64_t none = (2 * vb >= 64) ? ~0ull : ((1ull << (2 * vb)) - 1);
-    DBuf<uint64_t> hkeys(M);
-    DBuf<uint32_t> hvals(M);
+    tr.M = M;
+    tr.vb = vb;
+    tr.none = none;
+    tr.hkeys.alloc(M);
+    tr.hvals.alloc(M);
     hipLaunchKernelGGL(k_half_edges, dim3(blocks_for(M, kT)), dim3(kT), 0, st, d_F, nF, (const int32_t*)pos.p, vb, none,
-                       hkeys.p, hvals.p);
+                       tr.hkeys.p, tr.hvals.p);
     ITS_LAUNCH_CHECK();
-    radix_sort_pairs(hkeys, hvals, M, std::min(64, 2 * vb), st);
+    radix_sort_pairs(tr.hkeys, tr.hvals, M, std::min(64, 2 * vb), st);
+    tr.is_open.alloc(1);
+    ITS_HIP(hipMemsetAsync(tr.is_open.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_closed_check, dim3(blocks_for(M, kT)), dim3(kT), 0, st, (const uint64_t*)tr.hkeys.p,
+                       (const uint32_t*)tr.hvals.p, M, none, tr.is_open.p);
+    ITS_LAUNCH_CHECK();
+}
 
-    DBuf<int> is_open(1);
-    ITS_HIP(hipMemsetAsync(is_open.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_closed_check, dim3(blocks_for(M, kT)), dim3(kT), 0, st, (const uint64_t*)hkeys.p,
-                       (const uint32_t*)hvals.p, M, none, is_open.p);
-    ITS_LAUNCH_CHECK();
+// Boundary fans of the tree's nodes, for the winding-number walk; built only
+// when some point needs that walk.
+void fans_build(TriTree& tr, hipStream_t st, const double* d_V) {
+    const Shape& s = tr.s;
+    const int64_t M = tr.M;
+    const int vb = tr.vb;
+    const uint64_t none = tr.none;
+    const uint64_t* hkeys = tr.hkeys.p;
+    const uint32_t* hvals = tr.hvals.p;
     DBuf<uint32_t> node_cnt(tr.nodes);
     ITS_HIP(hipMemsetAsync(node_cnt.p, 0, tr.nodes * sizeof(uint32_t), st));
-    hipLaunchKernelGGL(k_fan_node_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
+    hipLaunchKernelGGL(k_fan_node_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys, hvals, M, s, none,
                        node_cnt.p);
     ITS_LAUNCH_CHECK();
     DBuf<uint8_t> keep(tr.nodes);
     {
         std::vector<uint32_t> h_cnt(tr.nodes);
-        int h_open = 1;
         ITS_HIP(hipMemcpyAsync(h_cnt.data(), node_cnt.p, tr.nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        ITS_HIP(hipMemcpyAsync(&h_open, is_open.p, sizeof(int), hipMemcpyDeviceToHost, st));
         ITS_HIP(hipStreamSynchronize(st));
-        tr.closed = h_open == 0;
         const std::vector<uint8_t> h_keep = fan_choice(s, std::min(kLeafLevel, (int)s.H), h_cnt);
         ITS_HIP(hipMemcpyAsync(keep.p, h_keep.data(), tr.nodes, hipMemcpyHostToDevice, st));
         ITS_HIP(hipStreamSynchronize(st));
     }
     DBuf<uint32_t> ecnt(M);
-    hipLaunchKernelGGL(k_fan_edge_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none,
+    hipLaunchKernelGGL(k_fan_edge_count, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys, hvals, M, s, none,
                        (const uint8_t*)keep.p, ecnt.p);
     ITS_LAUNCH_CHECK();
     const uint64_t E = scan_exclusive_u32(ecnt.p, ecnt.p, M, st);
@@ -671,7 +691,7 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     DBuf<uint32_t> fan_node(std::max<uint64_t>(E, 1));
     DBuf<uint64_t> fan_edge(std::max<uint64_t>(E, 1));
     if (E) {
-        hipLaunchKernelGGL(k_fan_emit, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys.p, hvals.p, M, s, none, vb,
+        hipLaunchKernelGGL(k_fan_emit, dim3(blocks_for(M, kT)), dim3(kT), 0, st, hkeys, hvals, M, s, none, vb,
                            (const uint8_t*)keep.p, (const uint32_t*)ecnt.p, fan_node.p, fan_edge.p);
         ITS_LAUNCH_CHECK();
         // stable: within a node, fan edges stay in edge-key order
@@ -874,7 +894,7 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
             const int64_t m = (a + b) >> 1;
             if (t.code[m] < c) a = m + 1; else b = m;
         }
-        const int64_t seed = std::min<int64_t>(a, t.s.T - 1) / ((int64_t)kBucket << leaf);
+        const int64_t seed = std::min<int64_t>(a, t.s.T - 1) >> (__builtin_ctz(kBucket) + leaf);
         leaf_nearest(t, leaf, seed, lane, p, pm, best, bt);
         // frontier: the nodes of the start level whose boxes are within the
         // seed's distance
@@ -1181,7 +1201,7 @@ __device__ inline bool box_on_ray(const double p[3], float4 lo, float4 hi, const
            pv <= (double)sel3(hi, R.v) && (R.sg > 0 ? pw <= (double)sel3(hi, R.w) : pw >= (double)sel3(lo, R.w));
 }
 __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, int64_t N, double* out,
-                                               uint8_t* decided) {
+                                               uint8_t* decided, unsigned* undecided) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kRayFront];
     __shared__ uint32_t leaves[kWavesPerBlock][kRayLeaves];
     const int lane = threadIdx.x & 63, w = wave_uniform((int)(threadIdx.x >> 6));
@@ -1270,6 +1290,7 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
         const int wn = wave_sum(cross);
         if (lane == 0) {
             decided[i] = unsure ? 0 : 1;
+            if (unsure) atomicAdd(undecided, 1u);
             if (!unsure && wn >= 1) out[i] = -out[i];   // winding number > 1/2
         }
     }
@@ -1369,16 +1390,26 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
     const int64_t wblocks = std::min<int64_t>((N + kWavesPerBlock - 1) / kWavesPerBlock, 8 * (int64_t)cus);
     hipLaunchKernelGGL(k_sdf_dist, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, d_tri);
     ITS_LAUNCH_CHECK();
-    // closed mesh: signs by the ray where it decides them, the fan walk for the rest
+    tr.closed = read1(tr.is_open.p, st) == 0;
+    // closed mesh: signs by the ray where it decides them; the fans are built
+    // and walked only for the points it leaves undecided
     DBuf<uint8_t> decided;
+    int64_t undecided = N;
     if (tr.closed && tune_get(ITS_TUNE_SDF_RAY)) {
         decided.alloc(N);
-        hipLaunchKernelGGL(k_sdf_ray, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, decided.p);
+        DBuf<unsigned> count(1);
+        ITS_HIP(hipMemsetAsync(count.p, 0, sizeof(unsigned), st));
+        hipLaunchKernelGGL(k_sdf_ray, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, decided.p,
+                           count.p);
+        ITS_LAUNCH_CHECK();
+        undecided = read1(count.p, st);
+    }
+    if (undecided > 0) {
+        fans_build(tr, st, d_V);
+        hipLaunchKernelGGL(k_sdf_wind, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
+                           (const uint8_t*)decided.p);
         ITS_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_sdf_wind, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
-                       (const uint8_t*)decided.p);
-    ITS_LAUNCH_CHECK();
     ITS_HIP(hipStreamSynchronize(st));
 }
 
