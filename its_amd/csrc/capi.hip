@@ -117,6 +117,7 @@ int its_ctx_destroy(its_ctx ctx) {
     if (!ctx) return ITS_OK;
     DeviceGuard g(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) block_cache_trim(ctx->stream);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -134,6 +135,7 @@ int its_ctx_set_stream(its_ctx ctx, void* hip_stream) {
     ITS_REQUIRE(ctx, ITS_E_INVALID, "null context");
     DeviceGuard g(ctx->device);
     ITS_HIP(hipStreamSynchronize(ctx->stream));
+    block_cache_trim(ctx->stream);   // blocks kept for the old stream
     if (ctx->own_stream) ITS_HIP(hipStreamDestroy(ctx->stream));
     if (hip_stream) {
         ctx->stream = (hipStream_t)hip_stream;

@@ -2,7 +2,9 @@
 // add back) and the rank directory of bitsets.  wave64 shuffles do the
 // intra-wave part; one block covers 2048 items.
 #include <atomic>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -43,6 +45,103 @@ void device_pool_trim() {
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return;
     hipMemPool_t pool = g_pools[dev].load(std::memory_order_acquire);
     if (pool) (void)hipMemPoolTrimTo(pool, 0);
+}
+
+namespace {
+struct BlockCache {
+    std::mutex mu;
+    std::unordered_map<void*, size_t> size_of;                    // blocks handed out by block_alloc: bytes
+    std::map<hipStream_t, std::multimap<size_t, void*>> free_of;  // per stream: bytes -> cached block
+    size_t cached = 0;                                            // bytes in the free lists
+};
+BlockCache g_blocks[kMaxDevices];
+
+size_t size_class(size_t b) {
+    if (b <= 256) return 256;
+    size_t p = 256;
+    while (p < b) p <<= 1;
+    if (p <= (size_t(1) << 20)) return p;
+    const size_t step = p >> 3;
+    return (b + step - 1) / step * step;
+}
+BlockCache* cache_here() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    return &g_blocks[dev];
+}
+}  // namespace
+
+void* block_alloc(size_t bytes, hipStream_t s, hipError_t* err) {
+    *err = hipSuccess;
+    const size_t want = size_class(bytes);
+    BlockCache* c = cache_here();
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto fl = c->free_of.find(s);
+        if (fl != c->free_of.end()) {
+            auto it = fl->second.lower_bound(want);
+            if (it != fl->second.end() && it->first <= 2 * want) {
+                void* p = it->second;
+                c->cached -= it->first;
+                fl->second.erase(it);
+                return p;
+            }
+        }
+    }
+    void* p = nullptr;
+    hipMemPool_t pool = device_pool();
+    hipError_t e = pool ? hipMallocFromPoolAsync(&p, want, pool, s) : hipMallocAsync(&p, want, s);
+    if (e != hipSuccess) {
+        *err = e;
+        return nullptr;
+    }
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->size_of[p] = want;
+    }
+    return p;
+}
+
+void block_free(void* p, hipStream_t s) {
+    BlockCache* c = cache_here();
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto it = c->size_of.find(p);
+        if (it != c->size_of.end() && c->cached + it->second <= kBlockCacheMax) {
+            c->free_of[s].emplace(it->second, p);
+            c->cached += it->second;
+            return;
+        }
+        if (it != c->size_of.end()) c->size_of.erase(it);
+    }
+    (void)hipFreeAsync(p, s);
+}
+
+void block_free_now(void* p) {
+    BlockCache* c = cache_here();
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->size_of.erase(p);
+    }
+    (void)hipFree(p);
+}
+
+void block_cache_trim(hipStream_t s) {
+    BlockCache* c = cache_here();
+    if (!c) return;
+    std::multimap<size_t, void*> blocks;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto fl = c->free_of.find(s);
+        if (fl == c->free_of.end()) return;
+        blocks.swap(fl->second);
+        c->free_of.erase(fl);
+        for (auto& b : blocks) {
+            c->cached -= b.first;
+            c->size_of.erase(b.second);
+        }
+    }
+    for (auto& b : blocks) (void)hipFree(b.second);
 }
 
 namespace {

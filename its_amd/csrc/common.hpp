@@ -51,6 +51,21 @@ hipMemPool_t device_pool();
 // give the unused blocks of the current device's pool back to the driver
 void device_pool_trim();
 
+// Block cache in front of the pool.  Stream-ordered frees and re-allocations
+// of the pool cost tens to hundreds of microseconds each, so a block freed
+// on a stream is kept in that stream's free list and handed to a later
+// allocation on the same stream (stream order makes the earlier uses finish
+// first, as with hipFreeAsync followed by hipMallocAsync).  Sizes are
+// rounded up to classes (powers of two up to 1 MiB, then eighths of the
+// power of two); a request takes a cached block of its class or of up to
+// twice its size.  At most kBlockCacheMax bytes per device are kept.
+constexpr size_t kBlockCacheMax = size_t(16) << 30;
+void* block_alloc(size_t bytes, hipStream_t s, hipError_t* err);   // s != nullptr
+void block_free(void* p, hipStream_t s);                            // s != nullptr: cached
+void block_free_now(void* p);                                       // hipFree, after a device wait
+// free the blocks cached for stream s (nothing may still use them: s idle)
+void block_cache_trim(hipStream_t s);
+
 inline hipStream_t& alloc_stream() {
     thread_local hipStream_t s = nullptr;
     return s;
@@ -84,9 +99,7 @@ struct DBuf {
             hipStream_t s = alloc_stream();
             hipError_t e;
             if (s) {
-                hipMemPool_t pool = device_pool();
-                e = pool ? hipMallocFromPoolAsync((void**)&p, count * sizeof(T), pool, s)
-                         : hipMallocAsync((void**)&p, count * sizeof(T), s);
+                p = (T*)block_alloc(count * sizeof(T), s, &e);
             } else {
                 e = hipMalloc(&p, count * sizeof(T));
             }
@@ -103,8 +116,8 @@ struct DBuf {
     void release() {
         if (p) {
             hipStream_t s = alloc_stream();
-            if (s) (void)hipFreeAsync(p, s);
-            else (void)hipFree(p);
+            if (s) block_free(p, s);
+            else block_free_now(p);
         }
         p = nullptr;
         n = 0;
