@@ -135,12 +135,24 @@ def main(argv=None):
     stats, t_fit = batch.fit(tol=1e-6)
     log(f"[batch] fit: one launch {t_fit * 1e3:.1f} ms vs mesh by mesh {seq_fit * 1e3:.1f} ms")
 
-    # evaluation fields with the collapse bound of each depth
+    # evaluation fields with the collapse bound of each depth: one
+    # its_batch_fields call per bound (octrees of different contexts are
+    # built side by side)
     s0 = time.perf_counter()
+    groups = {}
     for it in items:
-        if table:
-            autotune.apply(it["depth"], table)
-        it["g"].set_lambda(it["g"].get_lambda())
+        bound = (table[it["depth"]]["dense_cells"] if table and it["depth"] in table
+                 else _capi.tune_get(_capi.TUNE_DENSE_CELLS))
+        groups.setdefault(int(bound), []).append(it)
+    old_bound = _capi.tune_get(_capi.TUNE_DENSE_CELLS)
+    for bound, group in sorted(groups.items()):
+        _capi.tune_set(_capi.TUNE_DENSE_CELLS, bound)
+        fb = _capi.Batch(ctx)
+        for it in group:
+            fb.add(it["g"])
+        fb.fields()
+        fb.close()
+    _capi.tune_set(_capi.TUNE_DENSE_CELLS, old_bound)
     t_fields = time.perf_counter() - s0
     if table:
         autotune.apply(8, table)

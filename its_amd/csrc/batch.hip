@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <exception>
+#include <thread>
 #include <vector>
 
 #include "fit.hpp"
@@ -412,14 +414,46 @@ void batch_fit(its_batch b, double tol, int64_t max_iter, bool warm, its_lscg_st
     }
 }
 
-// rebuild the evaluation fields after a batch fit (per mesh)
+// Rebuild the evaluation fields after a batch fit, each on its octree's
+// stream, which it synchronises (its buffers and temporaries are allocated
+// and freed in that stream's order).  Octrees of different contexts are
+// independent: one host thread per context stream builds them side by side.
 void batch_fields(its_batch b) {
+    std::vector<hipStream_t> streams;
+    std::vector<std::vector<its_svo>> groups;
     for (its_svo s : b->svos) {
-        // each on its octree's stream, which it synchronises; its buffers and
-        // temporaries are allocated and freed in that stream's order
-        AllocStream as(s->ctx->stream);
-        field_build(s);
+        const size_t g = std::find(streams.begin(), streams.end(), s->ctx->stream) - streams.begin();
+        if (g == streams.size()) {
+            streams.push_back(s->ctx->stream);
+            groups.emplace_back();
+        }
+        groups[g].push_back(s);
     }
+    auto build = [](const std::vector<its_svo>& group) {
+        for (its_svo s : group) {
+            AllocStream as(s->ctx->stream);
+            field_build(s);
+        }
+    };
+    if (groups.size() == 1) {
+        build(groups[0]);
+        return;
+    }
+    const int device = b->ctx->device;
+    std::vector<std::exception_ptr> errs(groups.size());
+    std::vector<std::thread> threads;
+    for (size_t g = 0; g < groups.size(); ++g)
+        threads.emplace_back([&, g]() {
+            try {
+                ITS_HIP(hipSetDevice(device));
+                build(groups[g]);
+            } catch (...) {
+                errs[g] = std::current_exception();
+            }
+        });
+    for (std::thread& t : threads) t.join();
+    for (const std::exception_ptr& e : errs)
+        if (e) std::rethrow_exception(e);
 }
 
 int64_t batch_count(its_batch b) { return (int64_t)b->svos.size(); }
