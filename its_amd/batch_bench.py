@@ -6,13 +6,15 @@ octree depth 5-8 (R = 32..256) drawn per mesh; 65,536 uniform queries per mesh.
 
 Stages: octree build and signed-distance right-hand side per mesh, from
 several host threads with a context each (--build-threads); the fit of
-all meshes in one persistent launch (its_batch_fit, graph-captured); the
-evaluation fields per mesh with the autotuned collapse bound of its depth;
+all meshes (its_batch_fit: the smaller meshes in one persistent
+graph-captured launch, the larger ones beside it through the many-mesh
+solver, one launch per solver pass for all of them); the evaluation fields
+with the autotuned collapse bound of each depth (its_batch_fields);
 the evaluation of all meshes' queries in one launch (its_batch_eval_device).
 The same fits and evaluations also run mesh by mesh through the single-mesh
 entry points for comparison.  Prints one JSON line.
 
-    python -m its_amd.batch_bench [--meshes 256] [--queries 65536] [--build-threads 4]
+    python -m its_amd.batch_bench [--meshes 256] [--queries 65536] [--build-threads 4] [--caps 1,65536]
 """
 import argparse
 import json
@@ -59,6 +61,9 @@ def main(argv=None):
     ap.add_argument("--table", default=os.path.join(HERE, "data", "autotune.json"))
     ap.add_argument("--build-threads", type=int, default=4,
                     help="host threads building octrees and right-hand sides, one context each")
+    ap.add_argument("--caps", default="",
+                    help="comma-separated ITS_TUNE_BATCH_MAX_VERTS values: the whole fit is timed once more "
+                         "at each (meshes above the cap go to the many-mesh solver)")
     a = ap.parse_args(argv)
     import torch
     torch.cuda.init()
@@ -189,6 +194,24 @@ def main(argv=None):
     same = bool(torch.equal(out_b, out_s))
     log(f"[batch] eval: one launch {t_eval * 1e3:.2f} ms vs mesh by mesh {t_seq_eval * 1e3:.2f} ms (same {same})")
 
+    # the fit again at other one-launch caps (best of three after a capture)
+    cap_sweep = {}
+    if a.caps:
+        cap0 = _capi.tune_get(_capi.TUNE_BATCH_MAX_VERTS)
+        for cap in [int(c) for c in a.caps.split(",") if c]:
+            _capi.tune_set(_capi.TUNE_BATCH_MAX_VERTS, cap)
+            cb = _capi.Batch(ctx)
+            for it in items:
+                cb.add(it["g"])
+            cb.fit(tol=1e-6)
+            best = min(cb.fit(tol=1e-6)[1] for _ in range(3))
+            cb.close()
+            cap_sweep[cap] = {"fit_s": best, "one_launch_meshes": int(sum(1 for it in items
+                                                                            if it["g"].num_verts <= cap))}
+            log(f"[batch] cap {cap}: fit {best * 1e3:.1f} ms ({cap_sweep[cap]['one_launch_meshes']} meshes "
+                f"in the one-launch kernel)")
+        _capi.tune_set(_capi.TUNE_BATCH_MAX_VERTS, cap0)
+
     n_tot = sum(it["g"].num_verts for it in items)
     gpu_total = t_fit + t_eval
     res = {
@@ -207,6 +230,10 @@ def main(argv=None):
         "meshes_per_s_end_to_end": a.meshes / (t_build_wall + t_fit + t_fields + t_eval),
         "latency_ms_per_mesh_end_to_end": (t_build_wall + t_fit + t_fields + t_eval) / a.meshes * 1e3,
         "autotune_table": os.path.basename(a.table) if table else None,
+        "one_launch_cap": _capi.tune_get(_capi.TUNE_BATCH_MAX_VERTS),
+        "one_launch_meshes": int(sum(1 for it in items
+                                     if it["g"].num_verts <= _capi.tune_get(_capi.TUNE_BATCH_MAX_VERTS))),
+        "cap_sweep": cap_sweep,
     }
     print(json.dumps(res))
     batch.close()
