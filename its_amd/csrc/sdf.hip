@@ -48,7 +48,6 @@
 //     Geometric Predicates, 1997); a point with any undecided orientation
 //     (on or within rounding of an edge line or a triangle plane) takes the
 //     fan walk above.
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -283,7 +282,8 @@ __global__ void k_centroid_box(const double* V, const int32_t* F, int64_t nF, do
     if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_tri_keys(const double* V, const int32_t* F, int64_t nF, Quant q, int tb, uint64_t* keys) {
+// centroid code of every triangle, with its index as the sort's value
+__global__ void k_tri_keys(const double* V, const int32_t* F, int64_t nF, Quant q, uint32_t* code, uint32_t* idx) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nF) return;
     const double* a = V + 3 * (int64_t)F[3 * t];
@@ -291,22 +291,21 @@ __global__ void k_tri_keys(const double* V, const int32_t* F, int64_t nF, Quant 
     const double* c = V + 3 * (int64_t)F[3 * t + 2];
     double m[3];
     for (int x = 0; x < 3; ++x) m[x] = (a[x] + b[x] + c[x]) / 3.0;
-    keys[t] = ((uint64_t)quant_code(q, m) << tb) | (uint64_t)t;
+    code[t] = quant_code(q, m);
+    idx[t] = (uint32_t)t;
 }
 
-__global__ void k_tri_gather(const uint64_t* keys, int tb, int64_t nF, const double* V, const int32_t* F, double* tris,
-                             float4* sph, int32_t* tid, uint32_t* code, int32_t* pos) {
+__global__ void k_tri_gather(const uint32_t* idx, int64_t nF, const double* V, const int32_t* F, double* tris,
+                             float4* sph, int32_t* tid, int32_t* pos) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nF) return;
-    const uint64_t k = keys[i];
-    const int64_t t = (int64_t)(k & ((1ull << tb) - 1));
+    const int64_t t = (int64_t)idx[i];
     double T[9];
     for (int c = 0; c < 3; ++c)
         for (int a = 0; a < 3; ++a) T[3 * c + a] = V[3 * (int64_t)F[3 * t + c] + a];
     for (int j = 0; j < 9; ++j) tris[9 * i + j] = T[j];
     sph[i] = tri_sphere(T);
     tid[i] = (int32_t)t;
-    code[i] = (uint32_t)(k >> tb);
     pos[t] = (int32_t)i;
 }
 
@@ -517,25 +516,141 @@ int bits_for(uint64_t maxval) {
     return b;
 }
 
-template <class K, class Vv>
-void radix_sort_pairs(DBuf<K>& keys, DBuf<Vv>& vals, int64_t n, int bits, hipStream_t st) {
-    DBuf<K> k2(std::max<int64_t>(n, 1));
-    DBuf<Vv> v2(std::max<int64_t>(n, 1));
-    size_t tmp = 0;
-    ITS_HIP(rocprim::radix_sort_pairs(nullptr, tmp, keys.p, k2.p, vals.p, v2.p, (size_t)n, 0, bits, st));
-    DBuf<char> t(std::max<size_t>(tmp, 1));
-    ITS_HIP(rocprim::radix_sort_pairs(t.p, tmp, keys.p, k2.p, vals.p, v2.p, (size_t)n, 0, bits, st));
-    std::swap(keys, k2);
-    std::swap(vals, v2);
+// ---------------------------------------------------------- radix sort ----
+// Stable LSD radix sort on the low `bits` bits of the keys, 8-bit digits,
+// three launches per digit:
+//   - k_rs_hist: the digit histogram of each tile of 2048 keys (LDS
+//     atomics), stored digit-major, hist[d * tiles + tile];
+//   - k_rs_offsets: one workgroup per digit scans that digit's row of tile
+//     counts (exclusive) and stores the row total;
+//   - k_rs_scatter: each tile starts every digit at the sum of the smaller
+//     digits' totals plus its row offset, then, in 8 rounds of 256 keys in
+//     input order, ranks every key among the keys of its digit before it (a
+//     mask of the wavefront's lanes with the same digit from 8 ballots, and
+//     per-wave counts in LDS) and writes key and value there.
+// Keys with equal digits keep their input order within a tile, and tiles
+// follow one another in the offsets, so every pass is stable and equal keys
+// end in their input order, which the triangle order and the fan edge order
+// rely on.
+constexpr int kRsThreads = 256, kRsRounds = 8, kRsTile = kRsThreads * kRsRounds, kRsBins = 256;
+static_assert(kRsThreads == kRsBins, "one histogram bin per thread");
+
+template <class K>
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* keys, int64_t n, int shift, uint32_t* hist,
+                                                       int64_t tiles) {
+    __shared__ uint32_t h[kRsBins];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int j = 0; j < kRsRounds; ++j) {
+        const int64_t i = base + (int64_t)j * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (kRsBins - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-void radix_sort_keys(DBuf<uint64_t>& keys, int64_t n, int bits, hipStream_t st) {
-    DBuf<uint64_t> k2(std::max<int64_t>(n, 1));
-    size_t tmp = 0;
-    ITS_HIP(rocprim::radix_sort_keys(nullptr, tmp, keys.p, k2.p, (size_t)n, 0, bits, st));
-    DBuf<char> t(std::max<size_t>(tmp, 1));
-    ITS_HIP(rocprim::radix_sort_keys(t.p, tmp, keys.p, k2.p, (size_t)n, 0, bits, st));
-    std::swap(keys, k2);
+// exclusive scan of 256 values of a 256-thread block (valid in every thread)
+__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t* sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) sh[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int q = 0; q < w; ++q) before += sh[q];
+    __syncthreads();
+    return before + incl - v;
+}
+
+// row d = blockIdx.x of the digit-major tile counts -> exclusive offsets
+// within the digit; the row total to tot[d]
+__global__ __launch_bounds__(kRsThreads) void k_rs_offsets(uint32_t* hist, int64_t tiles, uint32_t* tot) {
+    __shared__ uint32_t sh[kRsThreads / 64 + 1];
+    uint32_t* row = hist + (int64_t)blockIdx.x * tiles;
+    uint32_t carry = 0;
+    for (int64_t b0 = 0; b0 < tiles; b0 += kRsThreads) {
+        const int64_t b = b0 + threadIdx.x;
+        const uint32_t v = b < tiles ? row[b] : 0u;
+        const uint32_t ex = block_excl_scan256(v, sh);
+        if (b < tiles) row[b] = carry + ex;
+        if (threadIdx.x == kRsThreads - 1) sh[kRsThreads / 64] = ex + v;
+        __syncthreads();
+        carry += sh[kRsThreads / 64];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+template <class K, class V>
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* kin, const V* vin, K* kout, V* vout, int64_t n,
+                                                          int shift, const uint32_t* off, int64_t tiles,
+                                                          const uint32_t* tot) {
+    constexpr int kW = kRsThreads / 64;
+    __shared__ uint32_t next[kRsBins];       // this tile's next output position per digit
+    __shared__ uint32_t wcnt[kW][kRsBins];   // keys per digit in each wavefront of the current round
+    __shared__ uint32_t sh[kW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t before = (1ull << lane) - 1ull;
+    // digit d starts after every smaller digit's keys and this digit's keys of earlier tiles
+    const uint32_t digit_base = block_excl_scan256(tot[threadIdx.x], sh);
+    next[threadIdx.x] = digit_base + off[(int64_t)threadIdx.x * tiles + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int j = 0; j < kRsRounds; ++j) {
+        const int64_t i = base + (int64_t)j * kRsThreads + threadIdx.x;
+        const bool valid = i < n;
+        const K k = valid ? kin[i] : K(0);
+        const uint32_t d = (uint32_t)(k >> shift) & (kRsBins - 1);
+        uint64_t same = __ballot(valid);   // lanes holding a key with this lane's digit
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t m = __ballot(set);
+            same &= set ? m : ~m;
+        }
+        for (int q = threadIdx.x; q < kW * kRsBins; q += kRsThreads) (&wcnt[0][0])[q] = 0u;
+        __syncthreads();
+        const uint32_t rank = (uint32_t)__popcll(same & before);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(same);
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = next[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+        }
+        __syncthreads();
+        uint32_t round_total = 0;
+        for (int q = 0; q < kW; ++q) round_total += wcnt[q][threadIdx.x];
+        next[threadIdx.x] += round_total;
+        if (valid) {
+            kout[pos] = k;
+            if (vout) vout[pos] = vin[i];
+        }
+        __syncthreads();
+    }
+}
+
+// sorts keys (and vals, when given) in place; n below 2^32
+template <class K, class V>
+void radix_sort(DBuf<K>& keys, DBuf<V>* vals, int64_t n, int bits, hipStream_t st) {
+    if (n <= 1 || bits <= 0) return;
+    ITS_REQUIRE(n < (int64_t(1) << 32), ITS_E_RANGE, "radix sort of 2^32 keys or more");
+    const int64_t tiles = (n + kRsTile - 1) / kRsTile;
+    DBuf<K> k2(n);
+    DBuf<V> v2(vals ? n : 0);
+    DBuf<uint32_t> hist(kRsBins * tiles), tot(kRsBins);
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)tiles), dim3(kRsThreads), 0, st, (const K*)keys.p, n, shift,
+                           hist.p, tiles);
+        ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_rs_offsets, dim3(kRsBins), dim3(kRsThreads), 0, st, hist.p, tiles, tot.p);
+        ITS_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_rs_scatter<K, V>), dim3((unsigned)tiles), dim3(kRsThreads), 0, st, (const K*)keys.p,
+                           vals ? (const V*)vals->p : (const V*)nullptr, k2.p, vals ? v2.p : (V*)nullptr, n, shift,
+                           (const uint32_t*)hist.p, tiles, (const uint32_t*)tot.p);
+        ITS_LAUNCH_CHECK();
+        std::swap(keys, k2);
+        if (vals) std::swap(*vals, v2);
+    }
 }
 
 struct TriTree {
@@ -605,19 +720,18 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
         tr.q.scale[a] = (hi > lo) ? 1024.0 / (hi - lo) : 0.0;
     }
 
-    // sort triangles by centroid code (ties by index)
-    const int tb = bits_for((uint64_t)(nF - 1));
-    DBuf<uint64_t> tkeys(nF);
-    hipLaunchKernelGGL(k_tri_keys, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, d_V, d_F, nF, tr.q, tb, tkeys.p);
+    // sort triangles by centroid code (stable: ties by index)
+    tr.code.alloc(nF);
+    DBuf<uint32_t> tidx(nF);
+    hipLaunchKernelGGL(k_tri_keys, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, d_V, d_F, nF, tr.q, tr.code.p, tidx.p);
     ITS_LAUNCH_CHECK();
-    radix_sort_keys(tkeys, nF, 30 + tb, st);
+    radix_sort(tr.code, &tidx, nF, 30, st);
     tr.tris.alloc(9 * (size_t)nF);
     tr.sph.alloc(nF);
     tr.tid.alloc(nF);
-    tr.code.alloc(nF);
     DBuf<int32_t> pos(nF);
-    hipLaunchKernelGGL(k_tri_gather, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, tkeys.p, tb, nF, d_V, d_F, tr.tris.p,
-                       tr.sph.p, tr.tid.p, tr.code.p, pos.p);
+    hipLaunchKernelGGL(k_tri_gather, dim3(blocks_for(nF, kT)), dim3(kT), 0, st, (const uint32_t*)tidx.p, nF, d_V, d_F,
+                       tr.tris.p, tr.sph.p, tr.tid.p, pos.p);
     ITS_LAUNCH_CHECK();
     // code-prefix table with about one prefix per triangle (3 to 21 bits)
     tr.cell_bits = 3;
@@ -650,7 +764,7 @@ void tree_build(TriTree& tr, hipStream_t st, const double* d_V, int64_t nV, cons
     hipLaunchKernelGGL(k_half_edges, dim3(blocks_for(M, kT)), dim3(kT), 0, st, d_F, nF, (const int32_t*)pos.p, vb, none,
                        tr.hkeys.p, tr.hvals.p);
     ITS_LAUNCH_CHECK();
-    radix_sort_pairs(tr.hkeys, tr.hvals, M, std::min(64, 2 * vb), st);
+    radix_sort(tr.hkeys, &tr.hvals, M, std::min(64, 2 * vb), st);
     tr.is_open.alloc(1);
     ITS_HIP(hipMemsetAsync(tr.is_open.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_closed_check, dim3(blocks_for(M, kT)), dim3(kT), 0, st, (const uint64_t*)tr.hkeys.p,
@@ -695,7 +809,7 @@ void fans_build(TriTree& tr, hipStream_t st, const double* d_V) {
                            (const uint8_t*)keep.p, (const uint32_t*)ecnt.p, fan_node.p, fan_edge.p);
         ITS_LAUNCH_CHECK();
         // stable: within a node, fan edges stay in edge-key order
-        radix_sort_pairs(fan_node, fan_edge, (int64_t)E, bits_for((uint64_t)tr.nodes), st);
+        radix_sort(fan_node, &fan_edge, (int64_t)E, bits_for((uint64_t)tr.nodes), st);
     }
     DBuf<uint32_t> fan_off(tr.nodes);
     hipLaunchKernelGGL(k_fan_kept_count, dim3(blocks_for(tr.nodes, kT)), dim3(kT), 0, st, tr.nodes,
