@@ -4,8 +4,9 @@ Workload (batching): 256 seeded meshes -- noisy icospheres, tori and
 blob unions in turn -- with triangle counts log-uniform in [1k, 200k] and
 octree depth 5-8 (R = 32..256) drawn per mesh; 65,536 uniform queries per mesh.
 
-Stages: octree build and signed-distance right-hand side per mesh, from
-several host threads with a context each (--build-threads); the fit of
+Stages: octree build, signed-distance right-hand side and the solver's
+lists and preconditioner per mesh, from several host threads with a
+context each (--build-threads); the fit of
 all meshes (its_batch_fit: the smaller meshes in one persistent
 graph-captured launch, the larger ones beside it through the many-mesh
 solver, one launch per solver pass for all of them); the evaluation fields
@@ -85,7 +86,7 @@ def main(argv=None):
     nthr = max(1, a.build_threads)
     bctx = [ctx] if nthr == 1 else [_capi.Context(0) for _ in range(nthr)]
     items = [None] * a.meshes
-    times = np.zeros((a.meshes, 2))
+    times = np.zeros((a.meshes, 3))
     lock = threading.Lock()
     nxt = [0]
     errors = []
@@ -106,7 +107,9 @@ def main(argv=None):
                 s1 = time.perf_counter()
                 b = g.sdf_at_vertices(V, F)
                 s2 = time.perf_counter()
-                times[k] = (s1 - s0, s2 - s1)
+                g.lscg_structure()                   # the solver's lists and preconditioner
+                s3 = time.perf_counter()
+                times[k] = (s1 - s0, s2 - s1, s3 - s2)
                 items[k] = dict(kind=kind, T=len(F), depth=depth, g=g, b=b, origin=origin, width=width)
         except Exception as e:   # re-raised below
             errors.append(e)
@@ -120,9 +123,9 @@ def main(argv=None):
     t_build_wall = time.perf_counter() - w0
     if errors:
         raise errors[0]
-    t_build, t_rhs = float(times[:, 0].sum()), float(times[:, 1].sum())
+    t_build, t_rhs, t_setup = (float(times[:, c].sum()) for c in range(3))
     log(f"[batch] built {a.meshes} on {nthr} threads in {t_build_wall:.2f} s (summed per-call: octrees "
-        f"{t_build:.2f} s, right-hand sides {t_rhs:.2f} s)")
+        f"{t_build:.2f} s, right-hand sides {t_rhs:.2f} s, solver setup {t_setup:.2f} s)")
 
     # fit: mesh by mesh, then all in one launch
     for it in items:
@@ -225,7 +228,7 @@ def main(argv=None):
         "all_converged": bool(all(s["converged"] for s in stats)),
         "max_iterations": int(max(s["iterations"] for s in stats)),
         "build_threads": nthr, "build_rhs_wall_s": t_build_wall,
-        "build_s_summed": t_build, "rhs_s_summed": t_rhs, "fields_s": t_fields,
+        "build_s_summed": t_build, "rhs_s_summed": t_rhs, "solver_setup_s_summed": t_setup, "fields_s": t_fields,
         "meshes_per_s_fit_eval": a.meshes / gpu_total,
         "meshes_per_s_end_to_end": a.meshes / (t_build_wall + t_fit + t_fields + t_eval),
         "latency_ms_per_mesh_end_to_end": (t_build_wall + t_fit + t_fields + t_eval) / a.meshes * 1e3,
