@@ -264,6 +264,18 @@ void rank_build_async(RankWord* g, int64_t nwords, hipStream_t s, uint32_t* d_to
     const uint32_t* t = scan_launch<uint32_t>(PopcLoad{g}, RankStore{g}, RankLoad{g}, nwords, s, keep);
     ITS_HIP(hipMemcpyAsync(d_total, t, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
 }
+void scan_exclusive_u32_async(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, uint32_t* d_total) {
+    if (n <= 0) {
+        ITS_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
+        return;
+    }
+    // block-sum buffers: released as in rank_build_async
+    std::vector<DBuf<uint32_t>> keep;
+    keep.reserve(8);
+    const uint32_t* t =
+        scan_launch<uint32_t>(ArrLoad<uint32_t>{in}, ArrStore<uint32_t>{out}, ArrLoad<uint32_t>{out}, n, s, keep);
+    ITS_HIP(hipMemcpyAsync(d_total, t, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+}
 void scan_exclusive_u64_async(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s, uint64_t* d_total) {
     if (n <= 0) {
         ITS_HIP(hipMemsetAsync(d_total, 0, sizeof(uint64_t), s));

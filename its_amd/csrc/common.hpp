@@ -207,7 +207,8 @@ uint64_t scan_exclusive_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStr
 uint64_t rank_build(RankWord* g, int64_t nwords, hipStream_t s);
 // the same, leaving the total in device memory (no wait)
 void rank_build_async(RankWord* g, int64_t nwords, hipStream_t s, uint32_t* d_total);
-// The same scan with the total left in device memory at d_total; no wait.
+// The same scans with the total left in device memory at d_total; no wait.
+void scan_exclusive_u32_async(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, uint32_t* d_total);
 void scan_exclusive_u64_async(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s, uint64_t* d_total);
 
 template <class T>

@@ -114,6 +114,14 @@ __global__ void k_res_sort(int64_t n, const uint32_t* ptr, uint32_t* ridx) {
     }
 }
 
+// the list lengths into ctl[0] (parent entries) and ctl[1] (child entries);
+// error bit 4 when they differ
+__global__ void k_list_sizes(const uint32_t* npro, const uint32_t* nres, int* ctl) {
+    ctl[0] = (int)*npro;
+    ctl[1] = (int)*nres;
+    if (*npro != *nres) ctl[2] |= 4;
+}
+
 __global__ void k_updown(int64_t n, const uint8_t* lev, int L, const uint16_t* vx, const uint16_t* vy,
                          const uint16_t* vz, LatticeSet S, uint32_t* up, uint32_t* dn1) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -333,18 +341,20 @@ Fit* fit_get(its_svo s) {
         S.D[d] = (uint32_t)s->lv[d].R + 1;
         S.off[d] = (uint32_t)s->esum_verts[d];
     }
+    // ctl during the build: [0] parent entries, [1] child entries, [2] error
+    // bits, read back together at the end, the one wait of the build
     f->ctl.alloc(4);
     ITS_HIP(hipMemsetAsync(f->ctl.p, 0, 4 * sizeof(int), st));
     int* err = f->ctl.p + 2;
-    // prolongation lists
+    // prolongation lists: at most 8 parents per vertex; sized by that bound
+    // first, cut to their length once it is known
+    const int64_t cap = std::max<int64_t>(8 * n, 1);
     f->pro_ptr.alloc(n + 1);
     hipLaunchKernelGGL(k_pro_count, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
                        s->vz.p, f->pro_ptr.p);
     ITS_LAUNCH_CHECK();
-    int64_t npro = (int64_t)scan_exclusive_u32(f->pro_ptr.p, f->pro_ptr.p, n, st);
-    uint32_t npro32 = (uint32_t)npro;
-    ITS_HIP(hipMemcpyAsync(f->pro_ptr.p + n, &npro32, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    f->pro_idx.alloc(std::max<int64_t>(npro, 1));
+    scan_exclusive_u32_async(f->pro_ptr.p, f->pro_ptr.p, n, st, f->pro_ptr.p + n);
+    f->pro_idx.alloc(cap);
     hipLaunchKernelGGL(k_pro_fill, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, L, s->vx.p, s->vy.p,
                        s->vz.p, S, f->pro_ptr.p, f->pro_idx.p, err);
     ITS_LAUNCH_CHECK();
@@ -354,10 +364,8 @@ Fit* fit_get(its_svo s) {
     hipLaunchKernelGGL(k_res_count, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->pro_ptr.p, f->pro_idx.p,
                        f->res_ptr.p);
     ITS_LAUNCH_CHECK();
-    int64_t nres = (int64_t)scan_exclusive_u32(f->res_ptr.p, f->res_ptr.p, n, st);
-    ITS_REQUIRE(nres == npro, ITS_E_STATE, "restriction size mismatch");
-    ITS_HIP(hipMemcpyAsync(f->res_ptr.p + n, &npro32, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    f->res_idx.alloc(std::max<int64_t>(nres, 1));
+    scan_exclusive_u32_async(f->res_ptr.p, f->res_ptr.p, n, st, f->res_ptr.p + n);
+    f->res_idx.alloc(cap);
     {
         DBuf<uint32_t> cursor(n + 1);
         ITS_HIP(hipMemcpyAsync(cursor.p, f->res_ptr.p, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
@@ -366,8 +374,10 @@ Fit* fit_get(its_svo s) {
         ITS_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_res_sort, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, f->res_ptr.p, f->res_idx.p);
         ITS_LAUNCH_CHECK();
-        ITS_HIP(hipStreamSynchronize(st));
     }
+    hipLaunchKernelGGL(k_list_sizes, dim3(1), dim3(1), 0, st, (const uint32_t*)f->pro_ptr.p + n,
+                       (const uint32_t*)f->res_ptr.p + n, f->ctl.p);
+    ITS_LAUNCH_CHECK();
     // coincidence chains
     f->up.alloc(n);
     f->down.alloc(n);
@@ -394,10 +404,22 @@ Fit* fit_get(its_svo s) {
         f->invd32.alloc(n);
         hipLaunchKernelGGL(k_invdiag, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, acc.p, f->invd64.p, f->invd32.p);
         ITS_LAUNCH_CHECK();
-        ITS_HIP(hipStreamSynchronize(st));
     }
-    int e = read1(err, st);
-    ITS_REQUIRE(e == 0, ITS_E_STATE, "octree lattice is inconsistent (missing coarse corner vertex)");
+    int info[3] = {0, 0, 0};
+    ITS_HIP(hipMemcpyAsync(info, f->ctl.p, sizeof info, hipMemcpyDeviceToHost, st));
+    ITS_HIP(hipStreamSynchronize(st));
+    ITS_REQUIRE(info[2] == 0, ITS_E_STATE,
+                (info[2] & 4) ? "restriction size mismatch"
+                              : "octree lattice is inconsistent (missing coarse corner vertex)");
+    ITS_HIP(hipMemsetAsync(f->ctl.p, 0, 4 * sizeof(int), st));   // the solver's flags start cleared
+    // the lists at their length (the bound-sized buffers go back to the stream's cache)
+    for (auto* lst : {&f->pro_idx, &f->res_idx}) {
+        DBuf<uint32_t> exact(std::max<int64_t>((uint32_t)info[0], 1));
+        if (info[0] > 0)
+            ITS_HIP(hipMemcpyAsync(exact.p, lst->p, (size_t)(uint32_t)info[0] * sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, st));
+        std::swap(*lst, exact);
+    }
     for (auto* v : {&f->x, &f->r, &f->p, &f->z, &f->t, &f->g, &f->h}) v->alloc(n);
     for (auto* v : {&f->g64, &f->h64, &f->r64, &f->nr64, &f->b64, &f->tmp64}) v->alloc(n);
     f->part.alloc(2 * kRedBlocks);
