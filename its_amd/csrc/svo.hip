@@ -160,14 +160,19 @@ __global__ void k_pack_bits(const uint32_t* bits, int64_t nwords, RankWord* g) {
 }
 
 // ------------------------------------------------------------ levels -------
-__global__ void k_heads(const uint32_t* occ, int64_t n, uint32_t* head) {
+// The level loop launches over a bound of the level's cell count; the count
+// itself (*cnt) is in device memory.  Entries from the count to the bound
+// are zero, so the scan over the bound gives the parents' count.
+__global__ void k_heads(const uint32_t* occ, const uint32_t* cnt, int64_t bound, uint32_t* head) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    head[i] = (i == 0 || (occ[i] >> 3) != (occ[i - 1] >> 3)) ? 1u : 0u;
+    if (i >= bound) return;
+    const int64_t n = *cnt;
+    head[i] = (i < n && (i == 0 || (occ[i] >> 3) != (occ[i - 1] >> 3))) ? 1u : 0u;
 }
-__global__ void k_parents(const uint32_t* occ, int64_t n, uint32_t* prank, uint32_t* parents) {
+__global__ void k_parents(const uint32_t* occ, const uint32_t* cnt, int64_t bound, uint32_t* prank,
+                          uint32_t* parents) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= bound || i >= (int64_t)*cnt) return;
     uint32_t h = (i == 0 || (occ[i] >> 3) != (occ[i - 1] >> 3)) ? 1u : 0u;
     uint32_t r = prank[i] + h - 1u;   // exclusive count of heads + own head - 1
     prank[i] = r;
@@ -185,11 +190,12 @@ struct NodeOut {
 
 // nodes of one level: octets of the children of each occupied parent
 // (createNode_1/2 + createRemainNode, SVO.cu:560-672)
-__global__ void k_nodes_fill(int64_t n_nodes, const uint32_t* parents, uint32_t single_code, int64_t off, double gx,
-                             double gy, double gz, double w, NodeOut o) {
+// single_code: the root level's one cell code (device memory)
+__global__ void k_nodes_fill(int64_t n_nodes, const uint32_t* parents, const uint32_t* single_code, int64_t off,
+                             double gx, double gy, double gz, double w, NodeOut o) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n_nodes) return;
-    uint32_t m = (n_nodes == 1) ? single_code : ((parents[j >> 3] << 3) | (uint32_t)(j & 7));
+    uint32_t m = (n_nodes == 1) ? *single_code : ((parents[j >> 3] << 3) | (uint32_t)(j & 7));
     int64_t g = off + j;
     o.morton[g] = m;
     o.is_leaf[g] = 1;
@@ -384,31 +390,60 @@ void svo_build(its_svo s, const double* d_tris, int64_t T) {
     bits.release();
 
     // ---- level loop (SVO.cu:717-815): a level with a single occupied cell
-    // is the root
+    // is the root.  Every level up to the one-cell grid (R >> d == 1) is
+    // built with its buffers sized by a bound (the level below's bound, and
+    // the level's cell count); the cell counts stay in device memory and come
+    // back in one read, which places the root.  Levels above the root are
+    // dropped.
     int depth = 0;
-    double w = g.unit[0];
-    while (s->lv[depth].n_occ > 0) {
-        Level& L = s->lv[depth];
-        L.R = std::max(1, R >> depth);
-        L.width = w;
-        ++depth;
-        if (L.n_occ == 1) {
-            L.n_nodes = 1;
-            break;
+    DBuf<uint32_t> lcount;   // cells per level, device
+    if (l0.n_occ > 0) {
+        int top = 0;
+        while ((R >> top) > 1) ++top;
+        ITS_REQUIRE(top < kMaxLevels, ITS_E_RANGE, "octree deeper than supported");
+        lcount.alloc(top + 1);
+        const uint32_t n0 = (uint32_t)l0.n_occ;
+        ITS_HIP(hipMemcpyAsync(lcount.p, &n0, sizeof n0, hipMemcpyHostToDevice, st));
+        std::vector<int64_t> bound(top + 1);
+        bound[0] = l0.n_occ;
+        for (int d = 0; d < top; ++d) {
+            Level& L = s->lv[d];
+            Level& P = s->lv[d + 1];
+            const int64_t side = R >> (d + 1);
+            bound[d + 1] = std::min<int64_t>(bound[d], side * side * side);
+            L.prank.alloc(bound[d]);
+            hipLaunchKernelGGL(k_heads, dim3(blocks_for(bound[d], kT)), dim3(kT), 0, st, L.occ.p,
+                               (const uint32_t*)lcount.p + d, bound[d], L.prank.p);
+            ITS_LAUNCH_CHECK();
+            scan_exclusive_u32_async(L.prank.p, L.prank.p, bound[d], st, lcount.p + d + 1);
+            P.occ.alloc(bound[d + 1]);
+            hipLaunchKernelGGL(k_parents, dim3(blocks_for(bound[d], kT)), dim3(kT), 0, st, L.occ.p,
+                               (const uint32_t*)lcount.p + d, bound[d], L.prank.p, P.occ.p);
+            ITS_LAUNCH_CHECK();
         }
-        ITS_REQUIRE(depth < kMaxLevels, ITS_E_RANGE, "octree deeper than supported");
-        L.prank.alloc(L.n_occ);
-        hipLaunchKernelGGL(k_heads, dim3(blocks_for(L.n_occ, kT)), dim3(kT), 0, st, L.occ.p, L.n_occ, L.prank.p);
-        ITS_LAUNCH_CHECK();
-        int64_t np = (int64_t)scan_exclusive_u32(L.prank.p, L.prank.p, L.n_occ, st);
-        Level& P = s->lv[depth];
-        P.occ.alloc(np);
-        P.n_occ = np;
-        hipLaunchKernelGGL(k_parents, dim3(blocks_for(L.n_occ, kT)), dim3(kT), 0, st, L.occ.p, L.n_occ, L.prank.p,
-                           P.occ.p);
-        ITS_LAUNCH_CHECK();
-        L.n_nodes = 8 * np;
-        w *= 2.0;
+        std::vector<uint32_t> h;
+        to_host(h, lcount.p, top + 1, st);
+        double w = g.unit[0];
+        for (int d = 0; d <= top; ++d) {
+            Level& L = s->lv[d];
+            L.R = std::max(1, R >> d);
+            L.width = w;
+            L.n_occ = h[d];
+            depth = d + 1;
+            if (h[d] == 1) {
+                L.n_nodes = 1;
+                break;
+            }
+            ITS_REQUIRE(d < top && h[d] > 1, ITS_E_STATE, "octree level loop did not reach a single cell");
+            L.n_nodes = 8 * (int64_t)h[d + 1];
+            w *= 2.0;
+        }
+        for (int d = depth; d <= top; ++d) {   // above the root
+            s->lv[d].occ.release();
+            s->lv[d].prank.release();
+            s->lv[d].n_occ = 0;
+        }
+        s->lv[depth - 1].prank.release();
     }
     s->depth = depth;
     int64_t total = 0;
@@ -431,8 +466,8 @@ void svo_build(its_svo s, const double* d_tris, int64_t T) {
     for (int d = 0; d < depth; ++d) {
         Level& L = s->lv[d];
         const uint32_t* parents = (d + 1 < depth) ? s->lv[d + 1].occ.p : nullptr;
-        uint32_t single = (L.n_nodes == 1) ? read1(L.occ.p, st) : 0u;
-        hipLaunchKernelGGL(k_nodes_fill, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, parents, single,
+        hipLaunchKernelGGL(k_nodes_fill, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, parents,
+                           (const uint32_t*)L.occ.p,
                            s->esum_nodes[d], g.go[0], g.go[1], g.go[2], L.width, o);
         ITS_LAUNCH_CHECK();
     }
