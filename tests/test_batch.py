@@ -182,3 +182,37 @@ def test_batch_from_thread_contexts(ctx):
         g.close()
     for c_ in ctxs:
         c_.close()
+
+
+def test_many_mesh_warm_and_iteration_limit(ctx):
+    """With the one-launch cap at 1 every mesh goes through the many-mesh
+    solver.  Stopped by an iteration limit, and then restarted warm from
+    those coefficients, its coefficients and statistics equal the
+    single-mesh solver's bit for bit."""
+    from its_amd import _capi
+    items = _build(ctx)[:5]
+    old_cap = _capi.tune_get(_capi.TUNE_BATCH_MAX_VERTS)
+    _capi.tune_set(_capi.TUNE_BATCH_MAX_VERTS, 1)
+    keys = ("iterations", "refinements", "normal_rel", "residual_abs", "initial_normal_rel", "converged")
+    try:
+        single = [g.lscg(b, tol=1e-6, max_iter=5) for _, g, b in items]
+        lam0 = [lam.copy() for lam, _ in single]
+        warm = [g.lscg(b, x0=lam0[m], tol=1e-6) for m, (_, g, b) in enumerate(items)]
+        batch = _capi.Batch(ctx)
+        for _, g, _ in items:
+            batch.add(g)
+        st, _ = batch.fit(tol=1e-6, max_iter=5)
+        for m, (c, g, _) in enumerate(items):
+            assert np.array_equal(g.get_lambda(), single[m][0]), c
+            assert all(st[m][k] == single[m][1][k] for k in keys), (c, st[m], single[m][1])
+            assert st[m]["iterations"] <= 5
+        for m, (_, g, _) in enumerate(items):
+            g.set_lambda(lam0[m])
+        st_w, _ = batch.fit(tol=1e-6, warm=True)
+        for m, (c, g, _) in enumerate(items):
+            assert np.array_equal(g.get_lambda(), warm[m][0]), c
+            assert all(st_w[m][k] == warm[m][1][k] for k in keys), (c, st_w[m], warm[m][1])
+            assert st_w[m]["converged"]
+        batch.close()
+    finally:
+        _capi.tune_set(_capi.TUNE_BATCH_MAX_VERTS, old_cap)
