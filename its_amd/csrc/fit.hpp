@@ -224,11 +224,54 @@ __device__ inline bool stage_ptr(const Coarse& C, const uint32_t* ptr, uint32_t*
     __syncthreads();
     return true;
 }
-// g = G p on the coarse levels (with the search direction update when upd)
+// level of coarse vertex i (C.lc <= level < C.L)
+__device__ inline int coarse_level(const Coarse& C, uint32_t i) {
+    int d = C.lc;
+    while (d + 1 < C.L && i >= C.off[d + 1]) ++d;
+    return d;
+}
+
+// g = G p on the coarse levels (with the search direction update when upd).
+// When the range holds at most one vertex per thread, every thread loads its
+// vertex's list and values before the level loop, which then waits only for
+// the LDS values of the level above; the sums keep the same order.
 __device__ inline void prolong_coarse_body(const Coarse& C, const uint32_t* ptr, const uint32_t* idx, float* p,
                                            const float* z, float* g, const double* scal, int upd, float* sg,
                                            uint32_t* sptr) {
     const uint32_t base = C.off[C.lc];
+    if (C.off[C.L] - base <= blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool own = i < C.off[C.L];
+        const int lev = own ? coarse_level(C, i) : -1;
+        uint32_t id[8];
+        uint32_t k0 = 0, k1 = 0;
+        float pi = 0.f;
+        if (own) {
+            k0 = ptr[i];
+            k1 = ptr[i + 1];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) id[q] = (k0 + q < k1) ? idx[k0 + q] : base;
+            pi = p[i];
+            if (upd) {
+                pi = z[i] + (float)scal[S_BETA] * pi;
+                p[i] = pi;
+            }
+        }
+        for (int d = C.L - 1; d >= C.lc; --d) {
+            if (lev == d) {
+                float s = 0.f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (k0 + q < k1) s += sg[id[q] - base];
+                float w = (k1 > k0) ? 1.f / (float)(k1 - k0) : 0.f;
+                float v = pi + w * s;
+                sg[i - base] = v;
+                g[i] = v;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     const bool staged = stage_ptr(C, ptr, sptr);
     const float beta = upd ? (float)scal[S_BETA] : 0.f;
     for (int d = C.L - 1; d >= C.lc; --d) {
@@ -256,10 +299,44 @@ __device__ inline void prolong_coarse_body(const Coarse& C, const uint32_t* ptr,
         __syncthreads();
     }
 }
-// restriction over the coarse levels [C.lc, C.L) (C.lc >= 1)
+// restriction over the coarse levels [C.lc, C.L) (C.lc >= 1); one vertex
+// per thread when the range fits, as in prolong_coarse_body
 __device__ inline void restrict_coarse_body(const Coarse& C, const uint32_t* ptr, const uint32_t* ridx, float* h,
                                             float* sh, uint32_t* sptr) {
     const uint32_t base = C.off[C.lc];
+    if (C.off[C.L] - base <= blockDim.x) {
+        const uint32_t c = base + threadIdx.x;
+        const bool own = c < C.off[C.L];
+        const int lev = own ? coarse_level(C, c) : -1;
+        uint32_t v[32];   // child lists: at most 27 entries
+        uint32_t k0 = 0, k1 = 0;
+        float hc0 = 0.f, hlow[32];
+        if (own) {
+            k0 = ptr[c];
+            k1 = ptr[c + 1];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) v[q] = (k0 + q < k1) ? ridx[k0 + q] : base;
+            hc0 = h[c];
+            // children below the coarse range are final already: read them now
+#pragma unroll
+            for (int q = 0; q < 32; ++q) hlow[q] = (lev == C.lc && k0 + q < k1) ? h[v[q] & 0x0FFFFFFFu] : 0.f;
+        }
+        for (int d = C.lc; d < C.L; ++d) {
+            if (lev == d) {
+                float s = 0.f;
+#pragma unroll
+                for (int q = 0; q < 32; ++q) {
+                    const float hy = (d == C.lc) ? hlow[q] : ((k0 + q < k1) ? sh[(v[q] & 0x0FFFFFFFu) - base] : 0.f);
+                    if (k0 + q < k1) s += ldexpf(hy, -(int)(v[q] >> 28));
+                }
+                const float hc = hc0 + s;
+                sh[c - base] = hc;
+                h[c] = hc;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     const bool staged = stage_ptr(C, ptr, sptr);
     for (int d = C.lc; d < C.L; ++d) {
         for (uint32_t c = C.off[d] + threadIdx.x; c < C.off[d + 1]; c += blockDim.x) {
