@@ -85,9 +85,9 @@ __device__ inline void restrict_at(int64_t c, const uint32_t* ptr, const uint32_
     h[c] += gather_children(ridx, ptr[c], ptr[c + 1], (const T*)h);
 }
 // x += alpha p; r -= alpha t along the coincidence chain of i; h_i = (D^T r)_i
-// (alpha: the binary32 rounding of the step, used when do_update)
 __device__ inline void update_dt_at(int64_t i, const uint8_t* finest, const uint32_t* up, float* x, const float* p,
-                                    float* r, const float* t, float* h, float alpha, int do_update) {
+                                    float* r, const float* t, float* h, const double* scal, int do_update) {
+    float alpha = do_update ? (float)scal[S_ALPHA] : 0.f;
     if (do_update) x[i] = fmaf(alpha, p[i], x[i]);
     if (!finest[i]) {
         h[i] = 0.f;

@@ -225,34 +225,21 @@ __global__ __launch_bounds__(kT) void k_apply_down(int64_t n, const uint32_t* do
     __shared__ double sh[kT / 64];
     apply_down_block(n, blockIdx.x, gridDim.x, down, g, t, part, sh);
 }
-
-// x += alpha p; r -= alpha t along each coincidence chain; h = D^T r.  With
-// do_update, every block first forms alpha = absNew / ||t||^2 from the
-// partials of k_apply_down, in the order of the one-block sum (the same value
-// in every block; block 0 stores it).
-__global__ __launch_bounds__(kT) void k_update_dt(int64_t n, const uint8_t* finest, const uint32_t* up, float* x,
-                                                 const float* p, float* r, const float* t, float* h, double* scal,
-                                                 const int* ctl, int do_update, const double* part, unsigned np) {
+// alpha = absNew / ||t||^2
+__global__ __launch_bounds__(kT) void k_scalar_alpha(const double* part, unsigned np, double* scal, const int* ctl) {
     if (ctl[0]) return;
-    float alpha = 0.f;
-    if (do_update) {
-        __shared__ double sh[kT / 64];
-        __shared__ double s_alpha;
-        const double tt = sum_partials(part, np, 1, 0, sh);
-        if (threadIdx.x == 0) {
-            const double a = tt > 0 ? scal[S_ABSNEW] / tt : 0.0;
-            s_alpha = a;
-            if (blockIdx.x == 0) {
-                scal[S_TT] = tt;
-                scal[S_ALPHA] = a;
-            }
-        }
-        __syncthreads();
-        alpha = (float)s_alpha;
-    }
+    __shared__ double sh[kT / 64];
+    scalar_alpha_body(part, np, scal, sh);
+}
+
+// x += alpha p; r -= alpha t along each coincidence chain; h = D^T r
+__global__ __launch_bounds__(kT) void k_update_dt(int64_t n, const uint8_t* finest, const uint32_t* up, float* x,
+                                                 const float* p, float* r, const float* t, float* h, const double* scal,
+                                                 const int* ctl, int do_update) {
+    if (ctl[0]) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    update_dt_at(i, finest, up, x, p, r, t, h, alpha, do_update);
+    update_dt_at(i, finest, up, x, p, r, t, h, scal, do_update);
 }
 
 // h_c += sum_{y in res(c)} 2^-k(y) h_y for the vertices of one level
@@ -500,8 +487,7 @@ void dot64_pair(Fit* f, const double* a1, const double* b1, const double* a2, co
 // binary32 transposed pass on f->r into f->h (with optional fused update)
 void launch_AT32(Fit* f, hipStream_t st, int do_update, int check) {
     hipLaunchKernelGGL(k_update_dt, dim3(blocks_for(f->n, kT)), dim3(kT), 0, st, f->n, f->finest.p, f->up.p, f->x.p,
-                       f->p.p, f->r.p, f->t.p, f->h.p, f->scal.p, f->ctl.p, do_update, (const double*)f->part.p,
-                       red_blocks(f->n));
+                       f->p.p, f->r.p, f->t.p, f->h.p, f->scal.p, f->ctl.p, do_update);
     ITS_LAUNCH_CHECK();
     for (int d = 1; d < f->lc; ++d) {
         int64_t b = f->off[d], e = f->off[d + 1];
@@ -541,7 +527,9 @@ void launch_iteration(Fit* f, hipStream_t st) {
     }
     hipLaunchKernelGGL(k_apply_down, dim3(nb), dim3(kT), 0, st, n, f->down.p, f->g.p, f->t.p, f->part.p, f->ctl.p);
     ITS_LAUNCH_CHECK();
-    launch_AT32(f, st, 1, 1);   // alpha formed by the update's blocks
+    hipLaunchKernelGGL(k_scalar_alpha, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p, f->ctl.p);
+    ITS_LAUNCH_CHECK();
+    launch_AT32(f, st, 1, 1);
     hipLaunchKernelGGL(k_precond, dim3(nb), dim3(kT), 0, st, n, f->h.p, f->invd32.p, f->z.p, f->part.p, f->ctl.p);
     ITS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scalar_beta, dim3(1), dim3(kT), 0, st, (const double*)f->part.p, nb, f->scal.p, f->ctl.p, 0);

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kT) void kb_update_dt(const uint32_t* plan, const M
     if (M.ctl[0]) return;
     const int64_t i = B.lb * kT + threadIdx.x;
     if (i >= M.n) return;
-    update_dt_at(i, M.finest, M.up, M.x, M.p, M.r, M.t, M.h, do_update ? (float)M.scal[S_ALPHA] : 0.f, do_update);
+    update_dt_at(i, M.finest, M.up, M.x, M.p, M.r, M.t, M.h, M.scal, do_update);
 }
 __global__ __launch_bounds__(kT) void kb_restrict32(const uint32_t* plan, const MeshDesc* descs, int d) {
     const Blk B = resolve(plan, descs);
