@@ -286,15 +286,18 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo svo, its_mlp net, const double* b, d
 
 /* ---- many meshes per launch (no reference counterpart: the reference fits
  *      one model per process) ----
- * A batch holds octrees of one context, each with its right-hand side
+ * A batch holds octrees, each with its right-hand side
  * (its_sdf_at_vertices).  its_batch_fit solves the meshes with at most
  * ITS_TUNE_BATCH_MAX_VERTS unknowns in one persistent launch (one workgroup
  * per mesh runs that mesh's whole mixed-precision LSCG, the algorithm and
  * stopping rule of its_lscg_solve), captured once as a hipGraph and replayed
- * on later calls; larger meshes run at the same time through its_lscg_solve.
- * stats: one per mesh; seconds: wall time of the fit.  warm != 0 starts every mesh from its current coefficients.  The
- * coefficients stay on the octrees; their evaluation fields are rebuilt by
- * its_batch_fields.  its_batch_eval_device evaluates f for all meshes in one
+ * on later calls; the larger meshes run at the same time through the
+ * many-mesh solver (one launch per solver pass for all of them; per mesh the
+ * coefficients and statistics of its_lscg_solve, bit for bit).
+ * stats: one per mesh; seconds: wall time of the fit.  warm != 0 starts
+ * every mesh from its current coefficients.  The coefficients stay on the
+ * octrees; their evaluation fields are rebuilt by its_batch_fields (octrees
+ * of different contexts side by side).  its_batch_eval_device evaluates f for all meshes in one
  * launch: queries [offsets[m], offsets[m+1]) of the device SoA arrays belong
  * to mesh m (offsets: host, count + 1 values), support semantics inside each
  * octree's box, out_val outside. */

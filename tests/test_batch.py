@@ -34,7 +34,7 @@ def _build(ctx):
 
 @pytest.mark.parametrize("cap", [1 << 30, 3000])
 def test_batch_fit_and_eval_match_single(ctx, cap):
-    """cap: meshes above it take the multi-workgroup solver beside the launch."""
+    """cap: meshes above it take the many-mesh solver beside the one-launch kernel."""
     import torch
     from its_amd import _capi
     old_cap = _capi.tune_get(_capi.TUNE_BATCH_MAX_VERTS)
