@@ -198,24 +198,43 @@ __device__ inline float pow2neg(int d) { return __uint_as_float((uint32_t)(127 -
 // multiple of 32 ulps before the payload goes in (relative error at most
 // 2^-19), and readers clear the 5 bits.
 constexpr uint32_t kLinkBits = 31u;
-__device__ inline uint32_t link_decode(float4& b0, float4& b1, uint32_t& base) {
-    uint32_t u[8] = {__float_as_uint(b0.x), __float_as_uint(b0.y), __float_as_uint(b0.z), __float_as_uint(b0.w),
-                     __float_as_uint(b1.x), __float_as_uint(b1.y), __float_as_uint(b1.z), __float_as_uint(b1.w)};
-    uint32_t mask = 0u, bs = 0u;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        mask |= (u[k] & 1u) << k;
-        bs |= ((u[k] >> 1) & 15u) << (4 * k);
-        u[k] &= ~kLinkBits;
-    }
-    b0 = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]), __uint_as_float(u[3]));
-    b1 = make_float4(__uint_as_float(u[4]), __uint_as_float(u[5]), __uint_as_float(u[6]), __uint_as_float(u[7]));
-    base = bs;
-    return mask;
+// The low bytes of the 8 corner words, corner k in byte k (k < 4 in .x,
+// k >= 4 in .y): byte gathers, a few byte permutes.
+__device__ inline uint2 link_pack(float4 b0, float4 b1) {
+    return make_uint2((__float_as_uint(b0.x) & 0xFFu) | ((__float_as_uint(b0.y) & 0xFFu) << 8) |
+                          ((__float_as_uint(b0.z) & 0xFFu) << 16) | (__float_as_uint(b0.w) << 24),
+                      (__float_as_uint(b1.x) & 0xFFu) | ((__float_as_uint(b1.y) & 0xFFu) << 8) |
+                          ((__float_as_uint(b1.z) & 0xFFu) << 16) | (__float_as_uint(b1.w) << 24));
+}
+// mask bit of child `key` (0..7) from the packed bytes
+__device__ inline uint32_t link_has(uint2 pk, uint32_t key) {
+    return ((key < 4u ? pk.x : pk.y) >> (8u * (key & 3u))) & 1u;
+}
+// mask (bit k: bit 0 of byte k) and first child (nibble k: bits 1-4 of byte
+// k) from the packed bytes, four bytes per word at once
+__device__ inline uint32_t link_unpack(uint2 pk, uint32_t& base) {
+    auto m4 = [](uint32_t w) {
+        w &= 0x01010101u;
+        return (w | (w >> 7) | (w >> 14) | (w >> 21)) & 0xFu;
+    };
+    auto n4 = [](uint32_t w) {
+        w = (w >> 1) & 0x0F0F0F0Fu;
+        w |= w >> 4;
+        return (w & 0xFFu) | ((w >> 8) & 0xFF00u);
+    };
+    base = n4(pk.x) | (n4(pk.y) << 16);
+    return m4(pk.x) | (m4(pk.y) << 4);
 }
 __device__ inline float4 link_clear(float4 b) {
     return make_float4(__uint_as_float(__float_as_uint(b.x) & ~kLinkBits), __uint_as_float(__float_as_uint(b.y) & ~kLinkBits),
                        __uint_as_float(__float_as_uint(b.z) & ~kLinkBits), __uint_as_float(__float_as_uint(b.w) & ~kLinkBits));
+}
+// mask and first child of a linked block; its values with the payload cleared
+__device__ inline uint32_t link_decode(float4& b0, float4& b1, uint32_t& base) {
+    const uint32_t mask = link_unpack(link_pack(b0, b1), base);
+    b0 = link_clear(b0);
+    b1 = link_clear(b1);
+    return mask;
 }
 
 // every level, root -> leaf, clamped cells and hats: any point
@@ -438,7 +457,8 @@ __global__ __launch_bounds__(kT) void k_eval_batch(const EvalParams* P, const Ba
 // need be).
 //
 // Entries are the grid coordinates u, the partial sum, the collapsed cell's
-// link word and the result's place (slot, lane).  Points on the upper faces
+// packed link bytes and the result's place (slot, lane); only the queued
+// queries unpack the link into mask and first child.  Points on the upper faces
 // (u = R) take the clamped walk with everything outside the grid.
 constexpr int kWaves = kT / 64;
 constexpr int kQueue = 128;   // at most 63 pending + 64 new entries
@@ -446,7 +466,8 @@ constexpr int kSlots = 4;     // chunks waiting for fine walks, per wavefront
 
 struct WaveQueue {
     float ux[kWaves][kQueue], uy[kWaves][kQueue], uz[kWaves][kQueue], f[kWaves][kQueue];
-    uint32_t tag[kWaves][kQueue], base[kWaves][kQueue], mask[kWaves][kQueue];
+    uint32_t tag[kWaves][kQueue];
+    uint2 link[kWaves][kQueue];   // the collapsed cell's packed link bytes
     float res[kWaves][kSlots][64];
     int64_t start[kWaves][kSlots];   // first query of the chunk waiting in each slot
 };
@@ -458,11 +479,12 @@ __device__ inline void lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// fine levels lc-1 .. 0 for one queued query (base, mask: the collapsed
-// cell's link word)
-__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint32_t base,
-                                  uint32_t mask) {
+// fine levels lc-1 .. 0 for one queued query (lk: the collapsed cell's
+// packed link bytes)
+__device__ inline float fine_walk(const EvalParams& P, float ux, float uy, float uz, float f, uint2 lk) {
     const int cx = (int)ux, cy = (int)uy, cz = (int)uz;   // u >= 0: truncation is floor
+    uint32_t base;
+    const uint32_t mask = link_unpack(lk, base);
     uint32_t idx = child_index(base, mask, child_key(cx, cy, cz, P.lc - 1));
     for (int d = P.lc - 1; d >= 0; --d) {
         float4 b0 = P.blocks[d][2 * (size_t)idx];
@@ -495,8 +517,7 @@ __device__ inline void walk_head(const EvalParams& P, WaveQueue& Q, WaveState& S
     uint32_t slot = kSlots;   // no slot for idle lanes
     if ((uint32_t)lane < k) {
         const uint32_t pos = (S.head + lane) & (kQueue - 1);
-        const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.base[w][pos],
-                                  Q.mask[w][pos]);
+        const float r = fine_walk(P, Q.ux[w][pos], Q.uy[w][pos], Q.uz[w][pos], Q.f[w][pos], Q.link[w][pos]);
         const uint32_t t = Q.tag[w][pos];
         slot = t >> 6;
         Q.res[w][slot][t & 63] = r;
@@ -550,8 +571,11 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
             const uint32_t g = g_cell(P, cx, cy, cz);
             float4 g0 = P.G[2 * (size_t)g], g1 = P.G[2 * (size_t)g + 1];
             if (lc > 0) {
-                lk.y = link_decode(g0, g1, lk.x);
-                push = (lk.y >> child_key(cx, cy, cz, lc - 1)) & 1u;
+                // only the pushed queries' fine walks unpack the link
+                lk = link_pack(g0, g1);
+                push = link_has(lk, child_key(cx, cy, cz, lc - 1));
+                g0 = link_clear(g0);
+                g1 = link_clear(g1);
             }
             f = tri_level(g0, g1, ux, uy, uz, sc);
         }
@@ -566,8 +590,7 @@ __global__ __launch_bounds__(kT) void k_eval_q(Loader load, int64_t off, int64_t
                 Q.uz[w][pos] = uz;
                 Q.f[w][pos] = f;
                 Q.tag[w][pos] = ((uint32_t)cur << 6) | (uint32_t)lane;
-                Q.base[w][pos] = lk.x;
-                Q.mask[w][pos] = lk.y;
+                Q.link[w][pos] = lk;
             } else {
                 Q.res[w][cur][lane] = f;
             }
