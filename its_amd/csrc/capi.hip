@@ -1000,7 +1000,10 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, dou
     DBuf<uint16_t> x(16 * n);
     DBuf<float> scale(n), y(n);
     DBuf<double> w(n);
-    warm_features(s, x.p, scale.p, st);
+    DBuf<int> ferr(1);
+    // the features' structure check is read after the solve, so the
+    // prediction runs without a host round trip
+    warm_features(s, x.p, scale.p, st, ferr.p);
     mlp_forward(mlp_of(net), x.p, n, y.p, st);
     scale_to_f64(n, y.p, scale.p, w.p, st);
     warm_apply(s, w.p, s->lambda64.p, st);
@@ -1010,6 +1013,7 @@ int its_lscg_solve_mlp(its_ctx ctx, its_svo s, its_mlp net, const double* b, dou
     ITS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (seconds_predict) *seconds_predict = ms * 1e-3;
     lscg_solve(s, d_b, s->lambda64.p, tol, max_iter, true, stats);
+    ITS_REQUIRE(read1(ferr.p, st) == 0, ITS_E_STATE, "owner corner is not a vertex of its level");
     ITS_HIP(hipMemcpyAsync(lambda, s->lambda64.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
     ITS_HIP(hipStreamSynchronize(st));
     s->has_lambda = true;

@@ -164,7 +164,9 @@ void warm_dual(its_svo s, const double* d_lam, double* d_w);
 void warm_apply(its_svo s, const double* d_w, double* d_lam0, hipStream_t st);
 void lscg_diag(its_svo s, double* d_out, hipStream_t st);
 // warm.hip
-void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st);
+// d_err: when given, the structure check's flag is left there (cleared
+// first) for the caller to read later; otherwise it is read before returning
+void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st, int* d_err = nullptr);
 // mlp.hip
 struct Mlp;
 void mlp_init(Mlp& m, its_ctx ctx, uint64_t seed);

@@ -78,7 +78,7 @@ __global__ void k_features(int64_t n, const uint8_t* vlevel, const uint16_t* vx,
 }
 }  // namespace
 
-void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st) {
+void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st, int* d_err) {
     ITS_REQUIRE(s->has_rhs, ITS_E_STATE, "no right-hand side: call its_sdf_at_vertices first");
     const int64_t n = s->num_verts;
     if (n <= 0) return;
@@ -90,13 +90,17 @@ void warm_features(its_svo s, uint16_t* d_x, float* d_scale, hipStream_t st) {
         S.width[d] = s->voxel_width * (double)(1u << d);
     }
     AllocStream as(st);   // the flag lives on the stream that uses it
-    DBuf<int> err(1);
-    ITS_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    DBuf<int> err;
+    if (!d_err) {
+        err.alloc(1);
+        d_err = err.p;
+    }
+    ITS_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_features, dim3(blocks_for(n, kT)), dim3(kT), 0, st, n, s->vlevel.p, s->vx.p, s->vy.p, s->vz.p,
                        s->vowner.p, s->morton.p, s->node_origin.p, s->node_width.p, s->rhs.p, s->rhs_nrm.p, S,
-                       s->origin[0], s->origin[1], s->origin[2], s->width[0], d_x, d_scale, err.p);
+                       s->origin[0], s->origin[1], s->origin[2], s->width[0], d_x, d_scale, d_err);
     ITS_LAUNCH_CHECK();
-    ITS_REQUIRE(read1(err.p, st) == 0, ITS_E_STATE, "owner corner is not a vertex of its level");
+    if (err.p) ITS_REQUIRE(read1(err.p, st) == 0, ITS_E_STATE, "owner corner is not a vertex of its level");
 }
 
 }  // namespace its
