@@ -1497,12 +1497,22 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
     if (N <= 0) return;
     TriTree tr;
     tree_build(tr, st, d_V, nV, d_F, nF);
-    // one wavefront per point, grid-strided over 8 blocks per CU
+    // One wavefront per point, grid-strided over as many blocks as the CUs
+    // hold at once for that kernel: a block that waited for a free slot would
+    // start its whole share after the resident blocks ended theirs.
     int dev = 0, cus = 256;
     ITS_HIP(hipGetDevice(&dev));
     ITS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int64_t wblocks = std::min<int64_t>((N + kWavesPerBlock - 1) / kWavesPerBlock, 8 * (int64_t)cus);
-    hipLaunchKernelGGL(k_sdf_dist, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, d_tri);
+    auto grid_of = [&](const void* kernel) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kT, 0) != hipSuccess || per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        return (unsigned)std::min<int64_t>((N + kWavesPerBlock - 1) / kWavesPerBlock, (int64_t)per_cu * cus);
+    };
+    hipLaunchKernelGGL(k_sdf_dist, dim3(grid_of((const void*)k_sdf_dist)), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
+                       d_tri);
     ITS_LAUNCH_CHECK();
     tr.closed = read1(tr.is_open.p, st) == 0;
     // closed mesh: signs by the ray where it decides them; the fans are built
@@ -1513,14 +1523,14 @@ void signed_distance(hipStream_t st, const double* d_V, int64_t nV, const int32_
         decided.alloc(N);
         DBuf<unsigned> count(1);
         ITS_HIP(hipMemsetAsync(count.p, 0, sizeof(unsigned), st));
-        hipLaunchKernelGGL(k_sdf_ray, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out, decided.p,
-                           count.p);
+        hipLaunchKernelGGL(k_sdf_ray, dim3(grid_of((const void*)k_sdf_ray)), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
+                           decided.p, count.p);
         ITS_LAUNCH_CHECK();
         undecided = read1(count.p, st);
     }
     if (undecided > 0) {
         fans_build(tr, st, d_V);
-        hipLaunchKernelGGL(k_sdf_wind, dim3((unsigned)wblocks), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
+        hipLaunchKernelGGL(k_sdf_wind, dim3(grid_of((const void*)k_sdf_wind)), dim3(kT), 0, st, tr.view(), d_pts, N, d_out,
                            (const uint8_t*)decided.p);
         ITS_LAUNCH_CHECK();
     }
