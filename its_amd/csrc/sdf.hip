@@ -1318,6 +1318,7 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
                                                uint8_t* decided, unsigned* undecided) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kRayFront];
     __shared__ uint32_t leaves[kWavesPerBlock][kRayLeaves];
+    __shared__ uint32_t cq[kWavesPerBlock][128];   // candidate triangles (sorted positions)
     const int lane = threadIdx.x & 63, w = wave_uniform((int)(threadIdx.x >> 6));
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
@@ -1389,16 +1390,37 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
             cur ^= 1;
             n = m;
         }
+        // Candidates: the triangles of the reached leaves whose bounding
+        // spheres meet the ray are queued and tested 64 at a time, so the
+        // wavefront reads binary64 vertices and runs the crossing test once
+        // per 64 candidates rather than once per leaf.  The count is a sum of
+        // integers, so the order does not change it.
         int cross = 0;
         bool lane_unsure = false;
         if (!unsure) {
             const float pm = fmaxf(fabsf((float)p[0]), fmaxf(fabsf((float)p[1]), fabsf((float)p[2])));
+            int qn = 0;   // queued candidates, below 64 between leaves
             for (int l = 0; l < nl; ++l) {
                 const int64_t k = leaves[w][l];
                 const int64_t j = node_first(leaf, k) + lane;
-                if (j < node_last(t.s, leaf, k) && sphere_on_ray(t, j, p, pm, R))
-                    cross += tri_ray(t.tris + 9 * j, p, R, lane_unsure);
+                const bool pass = j < node_last(t.s, leaf, k) && sphere_on_ray(t, j, p, pm, R);
+                const uint64_t mp = __ballot(pass);
+                if (pass) cq[w][qn + __popcll(mp & below)] = (uint32_t)j;
+                qn += __popcll(mp);
+                if (qn >= 64) {
+                    lds_fence();
+                    cross += tri_ray(t.tris + 9 * (int64_t)cq[w][lane], p, R, lane_unsure);
+                    qn -= 64;
+                    const uint32_t rest = lane < qn ? cq[w][64 + lane] : 0u;
+                    lds_fence();
+                    if (lane < qn) cq[w][lane] = rest;
+                }
             }
+            if (qn > 0) {
+                lds_fence();
+                if (lane < qn) cross += tri_ray(t.tris + 9 * (int64_t)cq[w][lane], p, R, lane_unsure);
+            }
+            lds_fence();
         }
         unsure = unsure || __ballot(lane_unsure) != 0ull;
         const int wn = wave_sum(cross);
