@@ -967,6 +967,8 @@ __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
 // result is the exact (d², index) minimum.  A frontier that would overflow
 // LDS sends the point to the node-by-node walk.
 constexpr int kFront = 256;
+// candidate queue: below 64 entries between rounds, plus two leaves of 64
+constexpr int kCandQueue = 192;
 __device__ void dist_dfs(const TreeView& t, int leaf, int64_t seed, int lane, const double p[3], const float pf[3],
                          float pm, double& best, int32_t& bt) {
     int h = t.s.H, k = 0;
@@ -988,7 +990,7 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
                                                  int32_t* tri_out) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kFront];   // frontier node indices within their level
     __shared__ float flb[kWavesPerBlock][kFront];        // box lower bounds of the leaf-level frontier
-    __shared__ uint32_t cq[kWavesPerBlock][128];         // candidate triangles (sorted positions)
+    __shared__ uint32_t cq[kWavesPerBlock][kCandQueue];  // candidate triangles (sorted positions)
     const int lane = threadIdx.x & 63, w = wave_uniform((int)(threadIdx.x >> 6));
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
@@ -1094,22 +1096,32 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
             // 64 candidates rather than once per leaf.  A test against an
             // older, larger best only queues more, so the minimum is that of
             // the full search.
-            int qn = 0;   // queued candidates, below 64 between leaves
-            for (int j = 0; j < n; ++j) {
-                const int64_t k = fk[cur][w][j];
-                if (k == seed || lb_beyond(flb[w][j], best)) continue;
-                const int64_t ti = node_first(leaf, k) + lane;
-                const bool pass = ti < node_last(t.s, leaf, k) && sphere_within(t, ti, p, pm, best);
-                const uint64_t mp = __ballot(pass);
-                if (pass) cq[w][qn + __popcll(mp & below)] = (uint32_t)ti;
-                qn += __popcll(mp);
-                if (qn >= 64) {
+            // Two leaves per round, so each lane has two sphere reads in
+            // flight; both are culled against the best of the round's start.
+            int qn = 0;   // queued candidates, below 64 between rounds
+            for (int j = 0; j < n; j += 2) {
+                const int64_t k0 = fk[cur][w][j];
+                const int64_t k1 = (j + 1 < n) ? (int64_t)fk[cur][w][j + 1] : -1;
+                const bool use0 = !(k0 == seed || lb_beyond(flb[w][j], best));
+                const bool use1 = k1 >= 0 && !(k1 == seed || lb_beyond(flb[w][j + 1], best));
+                const int64_t t0 = use0 ? node_first(leaf, k0) + lane : 0;
+                const int64_t t1 = use1 ? node_first(leaf, k1) + lane : 0;
+                const bool pass0 = use0 && t0 < node_last(t.s, leaf, k0) && sphere_within(t, t0, p, pm, best);
+                const bool pass1 = use1 && t1 < node_last(t.s, leaf, k1) && sphere_within(t, t1, p, pm, best);
+                const uint64_t m0 = __ballot(pass0), m1 = __ballot(pass1);
+                if (pass0) cq[w][qn + __popcll(m0 & below)] = (uint32_t)t0;
+                qn += __popcll(m0);
+                if (pass1) cq[w][qn + __popcll(m1 & below)] = (uint32_t)t1;
+                qn += __popcll(m1);
+                while (qn >= 64) {
                     lds_fence();
                     tri_nearest(t, cq[w][lane], true, p, best, bt);
                     qn -= 64;
-                    const uint32_t rest = lane < qn ? cq[w][64 + lane] : 0u;
+                    const uint32_t rest0 = lane < qn ? cq[w][64 + lane] : 0u;
+                    const uint32_t rest1 = lane + 64 < qn ? cq[w][128 + lane] : 0u;
                     lds_fence();
-                    if (lane < qn) cq[w][lane] = rest;
+                    if (lane < qn) cq[w][lane] = rest0;
+                    if (lane + 64 < qn) cq[w][64 + lane] = rest1;
                 }
             }
             if (qn > 0) {
@@ -1318,7 +1330,7 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
                                                uint8_t* decided, unsigned* undecided) {
     __shared__ uint32_t fk[2][kWavesPerBlock][kRayFront];
     __shared__ uint32_t leaves[kWavesPerBlock][kRayLeaves];
-    __shared__ uint32_t cq[kWavesPerBlock][128];   // candidate triangles (sorted positions)
+    __shared__ uint32_t cq[kWavesPerBlock][kCandQueue];   // candidate triangles (sorted positions)
     const int lane = threadIdx.x & 63, w = wave_uniform((int)(threadIdx.x >> 6));
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
@@ -1399,21 +1411,28 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
         bool lane_unsure = false;
         if (!unsure) {
             const float pm = fmaxf(fabsf((float)p[0]), fmaxf(fabsf((float)p[1]), fabsf((float)p[2])));
-            int qn = 0;   // queued candidates, below 64 between leaves
-            for (int l = 0; l < nl; ++l) {
-                const int64_t k = leaves[w][l];
-                const int64_t j = node_first(leaf, k) + lane;
-                const bool pass = j < node_last(t.s, leaf, k) && sphere_on_ray(t, j, p, pm, R);
-                const uint64_t mp = __ballot(pass);
-                if (pass) cq[w][qn + __popcll(mp & below)] = (uint32_t)j;
-                qn += __popcll(mp);
-                if (qn >= 64) {
+            int qn = 0;   // queued candidates, below 64 between rounds (two leaves a round)
+            for (int l = 0; l < nl; l += 2) {
+                const int64_t k0 = leaves[w][l];
+                const int64_t k1 = (l + 1 < nl) ? (int64_t)leaves[w][l + 1] : -1;
+                const int64_t j0 = node_first(leaf, k0) + lane;
+                const int64_t j1 = k1 >= 0 ? node_first(leaf, k1) + lane : 0;
+                const bool pass0 = j0 < node_last(t.s, leaf, k0) && sphere_on_ray(t, j0, p, pm, R);
+                const bool pass1 = k1 >= 0 && j1 < node_last(t.s, leaf, k1) && sphere_on_ray(t, j1, p, pm, R);
+                const uint64_t m0 = __ballot(pass0), m1 = __ballot(pass1);
+                if (pass0) cq[w][qn + __popcll(m0 & below)] = (uint32_t)j0;
+                qn += __popcll(m0);
+                if (pass1) cq[w][qn + __popcll(m1 & below)] = (uint32_t)j1;
+                qn += __popcll(m1);
+                while (qn >= 64) {
                     lds_fence();
                     cross += tri_ray(t.tris + 9 * (int64_t)cq[w][lane], p, R, lane_unsure);
                     qn -= 64;
-                    const uint32_t rest = lane < qn ? cq[w][64 + lane] : 0u;
+                    const uint32_t rest0 = lane < qn ? cq[w][64 + lane] : 0u;
+                    const uint32_t rest1 = lane + 64 < qn ? cq[w][128 + lane] : 0u;
                     lds_fence();
-                    if (lane < qn) cq[w][lane] = rest;
+                    if (lane < qn) cq[w][lane] = rest0;
+                    if (lane + 64 < qn) cq[w][64 + lane] = rest1;
                 }
             }
             if (qn > 0) {
