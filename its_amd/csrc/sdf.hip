@@ -943,11 +943,14 @@ __device__ inline void lds_fence() {
 // boxes, so a node that the walk from the root would have dropped with an
 // ancestor only makes the frontier larger, never changes the result.
 constexpr int kStartNodes = 128;
-__device__ inline int start_level(const Shape& s, int leaf) {
+__device__ inline int start_level(const Shape& s, int leaf, int most = kStartNodes) {
     int h = s.H;
-    while (h > leaf && s.count[h - 1] <= kStartNodes) --h;
+    while (h > leaf && s.count[h - 1] <= most) --h;
     return h;
 }
+// the distance walk starts lower, at up to 256 nodes (four per lane): at most
+// kFront survive, and every point saves one dependent round
+constexpr int kDistStartNodes = 256;
 
 __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
     return p[0] < (double)lo.x || p[0] > (double)hi.x || p[1] < (double)lo.y || p[1] > (double)hi.y ||
@@ -995,7 +998,7 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
-    const int start = start_level(t.s, leaf), start_count = t.s.count[start];
+    const int start = start_level(t.s, leaf, kDistStartNodes), start_count = t.s.count[start];
     for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
         const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
@@ -1016,31 +1019,30 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
         // seed's distance
         int n = 0, cur = 0;
         {
-            float lb0 = 0.f, lb1 = 0.f;
-            bool keep0 = false, keep1 = false;
-            if (lane < start_count) {
-                const NodeRec r0 = load_node(t, t.s.off[start] + lane);
-                lb0 = box_lb(pf, pm, r0.lo, r0.hi);
-                keep0 = !lb_beyond(lb0, best);
+            static_assert(kDistStartNodes <= kFront && kDistStartNodes <= 4 * 64, "start nodes: four per lane");
+            float lb[4];
+            bool keep[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                lb[q] = 0.f;
+                keep[q] = false;
+                if (lane + 64 * q < start_count) {
+                    const NodeRec r = load_node(t, t.s.off[start] + lane + 64 * q);
+                    lb[q] = box_lb(pf, pm, r.lo, r.hi);
+                    keep[q] = !lb_beyond(lb[q], best);
+                }
             }
-            if (lane + 64 < start_count) {
-                const NodeRec r1 = load_node(t, t.s.off[start] + lane + 64);
-                lb1 = box_lb(pf, pm, r1.lo, r1.hi);
-                keep1 = !lb_beyond(lb1, best);
-            }
-            const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
             lds_fence();
-            if (keep0) {
-                const int pos = __popcll(m0 & below);
-                fk[0][w][pos] = (uint32_t)lane;
-                flb[w][pos] = lb0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint64_t m = __ballot(keep[q]);
+                if (keep[q]) {
+                    const int pos = n + __popcll(m & below);
+                    fk[0][w][pos] = (uint32_t)(lane + 64 * q);
+                    flb[w][pos] = lb[q];
+                }
+                n += __popcll(m);
             }
-            if (keep1) {
-                const int pos = __popcll(m0) + __popcll(m1 & below);
-                fk[0][w][pos] = (uint32_t)lane + 64u;
-                flb[w][pos] = lb1;
-            }
-            n = __popcll(m0) + __popcll(m1);
             lds_fence();
         }
         bool overflow = false;
