@@ -948,9 +948,9 @@ __device__ inline int start_level(const Shape& s, int leaf, int most = kStartNod
     while (h > leaf && s.count[h - 1] <= most) --h;
     return h;
 }
-// the distance walk starts lower, at up to 256 nodes (four per lane): at most
-// kFront survive, and every point saves one dependent round
-constexpr int kDistStartNodes = 256;
+// the distance walk starts lower, at up to 512 nodes (eight per lane): at most
+// kFront survive, and every point saves two dependent rounds
+constexpr int kDistStartNodes = 512;
 
 __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
     return p[0] < (double)lo.x || p[0] > (double)hi.x || p[1] < (double)lo.y || p[1] > (double)hi.y ||
@@ -969,7 +969,7 @@ __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
 // distance, which is no larger than the seed's, so it is never dropped: the
 // result is the exact (d², index) minimum.  A frontier that would overflow
 // LDS sends the point to the node-by-node walk.
-constexpr int kFront = 256;
+constexpr int kFront = 512;
 // candidate queue: below 64 entries between rounds, plus two leaves of 64
 constexpr int kCandQueue = 192;
 __device__ void dist_dfs(const TreeView& t, int leaf, int64_t seed, int lane, const double p[3], const float pf[3],
@@ -1019,11 +1019,12 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
         // seed's distance
         int n = 0, cur = 0;
         {
-            static_assert(kDistStartNodes <= kFront && kDistStartNodes <= 4 * 64, "start nodes: four per lane");
-            float lb[4];
-            bool keep[4];
+            constexpr int kQ = kDistStartNodes / 64;
+            static_assert(kDistStartNodes <= kFront && kDistStartNodes % 64 == 0, "start nodes: kQ per lane");
+            float lb[kQ];
+            bool keep[kQ];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < kQ; ++q) {
                 lb[q] = 0.f;
                 keep[q] = false;
                 if (lane + 64 * q < start_count) {
@@ -1034,7 +1035,7 @@ __global__ __launch_bounds__(kT) void k_sdf_dist(TreeView t, const double* pts, 
             }
             lds_fence();
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < kQ; ++q) {
                 const uint64_t m = __ballot(keep[q]);
                 if (keep[q]) {
                     const int pos = n + __popcll(m & below);
