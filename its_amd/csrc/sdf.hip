@@ -1323,7 +1323,9 @@ __device__ inline int tri_ray(const double* T, const double p[3], const Ray& R, 
 // one per lane.  decided[i] = 1 and the sign is applied when every
 // orientation was certain and the lists fitted; otherwise decided[i] = 0
 // and the fan walk takes the point.
-constexpr int kRayFront = 128, kRayLeaves = 64;
+constexpr int kRayFront = 256, kRayLeaves = 64;
+// the ray walk starts at up to 256 nodes (four per lane), one round lower
+constexpr int kRayStartNodes = 256;
 __device__ inline bool box_on_ray(const double p[3], float4 lo, float4 hi, const Ray& R) {
     const double pu = sel3(p, R.u), pv = sel3(p, R.v), pw = sel3(p, R.w);
     return pu >= (double)sel3(lo, R.u) && pu <= (double)sel3(hi, R.u) && pv >= (double)sel3(lo, R.v) &&
@@ -1338,7 +1340,7 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     const int leaf = std::min(kLeafLevel, (int)t.s.H);
-    const int start = start_level(t.s, leaf), start_count = t.s.count[start];
+    const int start = start_level(t.s, leaf, kRayStartNodes), start_count = t.s.count[start];
     const NodeRec root = load_node(t, t.s.off[t.s.H]);
     for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + w; i < N; i += nw) {
         const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
@@ -1346,20 +1348,24 @@ __global__ __launch_bounds__(kT) void k_sdf_ray(TreeView t, const double* pts, i
         const Ray R = choose_ray(p, root.lo, root.hi);
         int n = 0, cur = 0, nl = 0;
         {   // the nodes of the start level whose boxes meet the ray
-            bool keep0 = false, keep1 = false;
-            if (!unsure && lane < start_count) {
-                const NodeRec r0 = load_node(t, t.s.off[start] + lane);
-                keep0 = box_on_ray(p, r0.lo, r0.hi, R);
+            constexpr int kQ = kRayStartNodes / 64;
+            static_assert(kRayStartNodes <= kRayFront && kRayStartNodes % 64 == 0, "start nodes: kQ per lane");
+            bool keep[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                keep[q] = false;
+                if (!unsure && lane + 64 * q < start_count) {
+                    const NodeRec r = load_node(t, t.s.off[start] + lane + 64 * q);
+                    keep[q] = box_on_ray(p, r.lo, r.hi, R);
+                }
             }
-            if (!unsure && lane + 64 < start_count) {
-                const NodeRec r1 = load_node(t, t.s.off[start] + lane + 64);
-                keep1 = box_on_ray(p, r1.lo, r1.hi, R);
-            }
-            const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
             lds_fence();
-            if (keep0) fk[0][w][__popcll(m0 & below)] = (uint32_t)lane;
-            if (keep1) fk[0][w][__popcll(m0) + __popcll(m1 & below)] = (uint32_t)lane + 64u;
-            n = __popcll(m0) + __popcll(m1);
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const uint64_t m = __ballot(keep[q]);
+                if (keep[q]) fk[0][w][n + __popcll(m & below)] = (uint32_t)(lane + 64 * q);
+                n += __popcll(m);
+            }
             lds_fence();
         }
         for (int h = start; n > 0; --h) {
