@@ -937,19 +937,18 @@ __device__ inline void lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// The frontier walks start at the lowest level of at most 128 nodes (not
-// below the leaves), testing all its nodes at once, two per lane, instead of
-// descending from the root one level per round.  A box holds its children's
-// boxes, so a node that the walk from the root would have dropped with an
-// ancestor only makes the frontier larger, never changes the result.
-constexpr int kStartNodes = 128;
-__device__ inline int start_level(const Shape& s, int leaf, int most = kStartNodes) {
+// The frontier walks start at the lowest level of at most `most` nodes (not
+// below the leaves), testing all its nodes at once, several per lane,
+// instead of descending from the root one level per round.  A box holds its
+// children's boxes, so a node that the walk from the root would have dropped
+// with an ancestor only makes the frontier larger, never changes the result.
+__device__ inline int start_level(const Shape& s, int leaf, int most) {
     int h = s.H;
     while (h > leaf && s.count[h - 1] <= most) --h;
     return h;
 }
-// the distance walk starts lower, at up to 512 nodes (eight per lane): at most
-// kFront survive, and every point saves two dependent rounds
+// the distance walk starts at up to 512 nodes (eight per lane); at most
+// kFront of them survive
 constexpr int kDistStartNodes = 512;
 
 __device__ inline bool box_outside(const double p[3], float4 lo, float4 hi) {
