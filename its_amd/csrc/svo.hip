@@ -228,22 +228,24 @@ __global__ void k_nodes_occ(int64_t n_occ, const uint32_t* occ, const uint32_t* 
 __constant__ uint8_t c_lut_parent[8 * 27];
 __constant__ uint8_t c_lut_child[8 * 27];
 
+// One thread per (node, neighbour slot) of the level, so a wavefront stores
+// 64 consecutive entries of the 27-per-node table.
 __global__ void k_neighbors(int64_t n, int64_t off, const uint32_t* morton, const uint32_t* parent,
                             const uint32_t* childbase, uint32_t* nb) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int64_t g = off + i;
-    uint32_t p = parent[g];
-    int key = morton[g] & 7;
-    for (int k = 0; k < 27; ++k) {
-        uint32_t pn = nb[27 * (int64_t)p + c_lut_parent[key * 27 + k]];
-        uint32_t v = ITS_NONE;
-        if (pn != ITS_NONE) {
-            uint32_t cb = childbase[pn];
-            v = (cb == ITS_NONE) ? ITS_NONE : cb + c_lut_child[key * 27 + k];
-        }
-        nb[27 * g + k] = v;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 27 * n) return;
+    const int64_t i = t / 27;
+    const int k = (int)(t - 27 * i);
+    const int64_t g = off + i;
+    const uint32_t p = parent[g];
+    const int key = morton[g] & 7;
+    const uint32_t pn = nb[27 * (int64_t)p + c_lut_parent[key * 27 + k]];
+    uint32_t v = ITS_NONE;
+    if (pn != ITS_NONE) {
+        const uint32_t cb = childbase[pn];
+        v = (cb == ITS_NONE) ? ITS_NONE : cb + c_lut_child[key * 27 + k];
     }
+    nb[27 * g + k] = v;
 }
 
 __global__ void k_root_self(uint32_t* nb, int64_t top) { nb[27 * top + 13] = (uint32_t)top; }
@@ -486,7 +488,7 @@ void svo_build(its_svo s, const double* d_tris, int64_t T) {
     ITS_LAUNCH_CHECK();
     for (int d = depth - 2; d >= 0; --d) {
         Level& L = s->lv[d];
-        hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, s->esum_nodes[d],
+        hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(27 * L.n_nodes, kT)), dim3(kT), 0, st, L.n_nodes, s->esum_nodes[d],
                            s->morton.p, s->parent.p, s->childbase.p, s->neighbors.p);
         ITS_LAUNCH_CHECK();
     }
